@@ -1,0 +1,221 @@
+#!/usr/bin/env python3
+"""Benchmark: in-place 3D C2C forward FFT of a 1024^3 complex<float> grid on
+one MI355X through libpfx.so (one-rank fast path: one launch per axis).
+
+Prints ONE JSON line.  ``value`` is GFLOP/s = 5 N log2(N) / t per forward
+transform (N = 1024^3), inputs resident in HBM.  Also reported: achieved HBM
+TB/s on algorithmic bytes (2 * D * N * 8 B per transform), rel-L2 / max-abs of
+the FP32 result against the FP64 path on the same input, the roofline of the
+dominant stage kernel (HIP events on the launch stream), and a CPU baseline
+(the FP64 oracle on a bounded sample of the same per-axis transforms).
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-check]
+                  [--no-cpu] [--precision f32|f32_split_f16|f32_split_bf16]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "3D FFT GFLOP/s + HBM TB/s, 1024^3 fp32, rel-L2≤2e-6 vs FP64; split vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(n: int, frac_log2: int = 4):
+    """Time the FP64 oracle (restated FFTWBackend<double> batched plans) on a
+    bounded sample of the workload: for each of the three axes, 1/2^frac_log2
+    of that axis' n*n length-n transforms, laid out with that axis' stride
+    pattern in an [n/2^frac_log2 ... ] block.  Returns GFLOP/s."""
+    import numpy as np
+    from oracle import oracle as O
+    m = (n * n) >> frac_log2          # transforms per axis in the sample
+    lead = m // n                     # slices
+    buf = O.uniform_complex((m * n,), seed=12345)
+    threads = O.max_threads()
+    t0 = time.perf_counter()
+    O.c2c_batched(buf, n, m, 1, n, -1, threads)                  # axis 2: rows
+    for i in range(lead):                                        # axis 1: [lead, n, n]
+        sl = buf[i * n * n:(i + 1) * n * n]
+        O.c2c_batched(sl, n, n, n, 1, -1, threads)
+    O.c2c_batched(buf, n, m, m, 1, -1, threads)                  # axis 0: [n, lead, n]
+    dt = time.perf_counter() - t0
+    flops = 3 * m * 5.0 * n * math.log2(n)
+    return {"value": round(flops / dt / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"FP64 oracle (restated FFTWBackend<double> batched plans, OpenMP over "
+                       f"the batch): {m} of each axis' {n * n} length-{n} transforms, with "
+                       f"strides 1, {n} and {m} elements; {dt:.2f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=1024)
+    ap.add_argument("--precision", default="f32")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU only (got --gpus {args.gpus}); the "
+              "multi-GPU exchange is not part of it.", file=sys.stderr)
+        sys.exit(2)
+
+    import ctypes as C
+    import torch
+    import parafaft_amd as P
+    from parafaft_amd import _capi as A
+
+    n = args.n
+    D = 3
+    N = n ** D
+    prec = P.backend.precision_code(args.precision)
+    L = A.load()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+
+    log(f"[bench] allocating {N * 8 / 2**30:.1f} GiB complex<float> {n}^3")
+    g = torch.Generator(device=dev)
+    g.manual_seed(20241203)
+    x = (torch.rand((N, 2), device=dev, generator=g, dtype=torch.float32) * 2 - 1)
+    x = torch.view_as_complex(x).reshape(n, n, n).contiguous()
+
+    nd = C.c_void_p()
+    shape = (C.c_int * 3)(n, n, n)
+    A.check(L.pfx_ndplan_create(C.byref(nd), prec, A.PFX_C2C, 3, shape), "pfx_ndplan_create")
+    stages = []
+    for s in range(D):
+        h = C.c_void_p()
+        A.check(L.pfx_ndplan_stage(nd, s, C.byref(h)), "pfx_ndplan_stage")
+        stages.append(h.value)
+    names = []
+    for h in stages:
+        buf = C.create_string_buffer(256)
+        pb, th = C.c_int(), C.c_int()
+        L.pfx_plan_describe(h, buf, 256, C.byref(pb), C.byref(th))
+        names.append(f"{buf.value.decode()}/tile{pb.value}/thr{th.value}")
+
+    # ---- accuracy (outside the timed region): FP32 vs FP64 on the same input
+    acc = {}
+    if not args.no_check:
+        log("[bench] accuracy check vs FP64 path")
+        x64 = x.to(torch.complex128)
+        nd64 = C.c_void_p()
+        A.check(L.pfx_ndplan_create(C.byref(nd64), A.PFX_F64, A.PFX_C2C, 3, shape), "nd64")
+        y = x.clone()
+        A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, y.data_ptr(), 1.0, sp), "fwd32")
+        A.check(L.pfx_ndplan_execute(nd64.value, A.PFX_FORWARD, x64.data_ptr(), 1.0, sp), "fwd64")
+        torch.cuda.synchronize()
+        diff = (y.to(torch.complex128) - x64)
+        acc["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(diff) /
+                                      torch.linalg.vector_norm(x64))
+        acc["max_abs_vs_fp64"] = float(diff.abs().max())
+        acc["max_abs_rel_to_peak"] = acc["max_abs_vs_fp64"] / float(x64.abs().max())
+        del diff, y, x64
+        L.pfx_ndplan_destroy(nd64.value)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+
+    def step(events=None):
+        for s, h in enumerate(stages):
+            if events is not None:
+                events[s][0].record(stream)
+            rc = L.pfx_execute(h, A.PFX_FORWARD, x.data_ptr(), sp)
+            if rc:
+                A.check(rc, "pfx_execute")
+            if events is not None:
+                events[s][1].record(stream)
+
+    log(f"[bench] warmup {args.warmup}")
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+
+    # timed region: exactly K forward transforms, synchronised on both sides
+    ev = [[[torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)]
+           for _ in range(D)] for _ in range(args.steps)]
+    log(f"[bench] timing {args.steps} steps")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        step(ev[k])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    stage_ms = [sum(ev[k][s][0].elapsed_time(ev[k][s][1]) for k in range(args.steps)) / args.steps
+                for s in range(D)]
+    log(f"[bench] ms/step {ms_per_step:.3f}, stage ms {['%.3f' % t for t in stage_ms]}")
+
+    flops = 5.0 * N * math.log2(N)
+    gflops = flops / (ms_per_step * 1e-3) / 1e9
+    alg_bytes = 2.0 * D * N * 8
+    tbs = alg_bytes / (ms_per_step * 1e-3) / 1e12
+    dom = max(range(D), key=lambda s: stage_ms[s])
+    per_launch = 2.0 * N * 8
+    achieved = per_launch / (stage_ms[dom] * 1e-3) / 1e9
+
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_1024cube_f32.json")
+    if os.path.exists(tpath):
+        try:
+            with open(tpath) as f:
+                t = json.load(f)
+            traffic = t.get("stages", {}).get(str(dom), {}).get("hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    cpu = None
+    if not args.no_cpu:
+        log("[bench] CPU baseline (oracle sample)")
+        try:
+            cpu = cpu_baseline(n)
+        except Exception as e:  # the oracle is optional on a box without gcc
+            cpu = {"value": None, "unit": "GFLOP/s", "cores": 0, "kind": "port",
+                   "sample": f"oracle unavailable: {e}"}
+
+    out = {
+        "metric": METRIC,
+        "value": round(gflops, 2),
+        "unit": "GFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32" if prec in (A.PFX_F32,) else args.precision,
+        "data": "synthetic: iid Uniform[-1,1) re/im (seeded), resident in HBM",
+        "config": {"workload": f"3D C2C forward, in place, {n}^3 complex<float>, one GPU "
+                               "(one-rank fast path: one launch per axis)",
+                   "grid": [n, n, n], "precision": args.precision,
+                   "stage_kernels": names},
+        "hbm_tbs_algorithmic": round(tbs, 3),
+        "stage_ms": [round(t, 4) for t in stage_ms],
+        "accuracy": acc,
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": traffic, "kernel": names[dom],
+                     "algorithmic_bytes_per_launch": per_launch},
+        "cpu_baseline": cpu,
+    }
+    L.pfx_ndplan_destroy(nd.value)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

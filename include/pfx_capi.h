@@ -1,0 +1,203 @@
+/*
+ * pfx_capi.h -- C ABI of the MI355X (gfx950) FFT backend for ParaFaFT.
+ *
+ * Plain C: pointers, sizes and status codes only.  Every function returns 0
+ * (PFX_SUCCESS) or a pfx_status; pfx_error_string() names the code and
+ * pfx_last_error_message() gives the detail of the calling thread's last
+ * failure.  All device work is enqueued asynchronously on the given stream
+ * (a hipStream_t passed as void*; NULL means the null stream).
+ *
+ * Each entry point names the reference interface it replaces.  Reference
+ * paths are relative to the ParaFaFT source tree.
+ */
+#ifndef PFX_CAPI_H
+#define PFX_CAPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pfx_plan_t *pfx_plan;
+typedef struct pfx_ndplan_t *pfx_ndplan;
+typedef void *pfx_stream; /* hipStream_t */
+
+enum pfx_status {
+  PFX_SUCCESS = 0,
+  PFX_ERR_INVALID_ARG = 1,
+  PFX_ERR_UNSUPPORTED = 2,
+  PFX_ERR_HIP = 3,
+  PFX_ERR_ALLOC = 4,
+  PFX_ERR_INTERNAL = 5
+};
+
+enum pfx_precision {
+  PFX_F32 = 0,            /* complex<float>  / float   */
+  PFX_F64 = 1,            /* complex<double> / double  */
+  PFX_F32_SPLIT_F16 = 2,  /* complex<float> data, radix-16 sub-DFTs on FP16 MFMA, hi/lo split */
+  PFX_F32_SPLIT_BF16 = 3  /* complex<float> data, radix-16 sub-DFTs on BF16 MFMA, hi/lo split */
+};
+
+enum pfx_kind { PFX_C2C = 0, PFX_R2C = 1, PFX_C2R = 2 };
+
+enum pfx_direction { PFX_FORWARD = 0, PFX_BACKWARD = 1 };
+
+/* ---------------------------------------------------------------- errors */
+const char *pfx_error_string(int status);
+const char *pfx_last_error_message(void);
+/* Library version string. */
+const char *pfx_version(void);
+
+/* ------------------------------------------------ batched strided 1D plans */
+
+/*
+ * C2C plan: `batch` transforms of `length` complex elements; element j of
+ * transform b is data[b*dist + j*stride] (complex units).  Both directions
+ * share one plan.
+ * Replaces: HipFFTBackend::create_stage_plan -> hipfftPlanMany(Z2Z/C2C)
+ *   (backend/hipfft/fft_backend_hipfft.hpp:294-314); geometry as in
+ *   FFTWBackend::create_stage_plan (backend/fftw3/fft_backend_fftw.hpp:387-424).
+ */
+int pfx_plan_create_c2c(pfx_plan *plan, int precision, int64_t length, int64_t batch,
+                        int64_t stride, int64_t dist);
+
+/*
+ * Two-level C2C plan: transform b = o*inner_batch + i starts at
+ * data[o*outer_dist + i*dist].  One launch covers what the reference issues
+ * as outer_batch separate execute_stage calls on a middle axis.
+ * Replaces: the leading_size loop of ParaFaFT_C2C::perform_fft_on_buffer
+ *   (parafaft_c2c.hpp:875-896; parafaft_r2c.hpp:1230-1243).
+ */
+int pfx_plan_create_c2c_many(pfx_plan *plan, int precision, int64_t length,
+                             int64_t inner_batch, int64_t stride, int64_t dist,
+                             int64_t outer_batch, int64_t outer_dist);
+
+/*
+ * In-place padded R2C plan: row b starts at scalar b*dist_scalars; its
+ * `length` reals are replaced by length/2+1 complex values (complex row
+ * distance dist_scalars/2).  stride must be 1 (the only value ParaFaFT uses).
+ * Replaces: HipFFTBackend::create_r2c_inplace_plan (fft_backend_hipfft.hpp:361-384),
+ *   FFTWBackend::create_r2c_inplace_plan (fft_backend_fftw.hpp:439-469).
+ */
+int pfx_plan_create_r2c(pfx_plan *plan, int precision, int64_t length, int64_t batch,
+                        int64_t stride, int64_t dist_scalars);
+
+/*
+ * In-place padded C2R plan (unnormalised), mirror of the R2C plan.
+ * Replaces: HipFFTBackend::create_c2r_inplace_plan (fft_backend_hipfft.hpp:426-447),
+ *   FFTWBackend::create_c2r_inplace_plan (fft_backend_fftw.hpp:503-530).
+ */
+int pfx_plan_create_c2r(pfx_plan *plan, int precision, int64_t length, int64_t batch,
+                        int64_t stride, int64_t dist_scalars);
+
+/*
+ * Execute a plan in place on `data` (any element-aligned device pointer with
+ * the plan's layout: the new-array execute semantics of
+ * backend/fft_backend.hpp:107-111).  direction: PFX_FORWARD = exp(-2 pi i jk/n),
+ * PFX_BACKWARD = exp(+2 pi i jk/n); both unnormalised.  R2C plans accept only
+ * PFX_FORWARD, C2R plans only PFX_BACKWARD.
+ * Replaces: HipFFTBackend::execute_stage / execute_r2c_inplace /
+ *   execute_c2r_inplace (fft_backend_hipfft.hpp:326-339, 396-405, 459-467).
+ */
+int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream);
+
+/*
+ * As pfx_execute, with every output multiplied by `scale` inside the last
+ * pass (the user-side 1/N normalisation loop of example/c2c.cpp:27-31 fused
+ * into the transform).
+ */
+int pfx_execute_scaled(pfx_plan plan, int direction, void *data, double scale,
+                       pfx_stream stream);
+
+int pfx_plan_destroy(pfx_plan plan);
+
+/* Kernel chosen for a plan: name buffer, transforms per workgroup, threads
+ * per workgroup.  Diagnostics only. */
+int pfx_plan_describe(pfx_plan plan, char *name, size_t name_len, int *per_block,
+                      int *threads);
+
+/* --------------------------------------------------------------- memory */
+
+/* Replaces: hipvector<T>::resize / ~hipvector (fft_backend_hipfft.hpp:58-121). */
+int pfx_malloc(void **ptr, size_t bytes);
+int pfx_free(void *ptr);
+
+/* Replaces: HipFFTBackend::memcpy (fft_backend_hipfft.hpp:481-484). */
+int pfx_memcpy_async(void *dst, const void *src, size_t bytes, pfx_stream stream);
+
+/*
+ * Strided 2D copy: `height` rows of `width` bytes.  Device-to-device copies
+ * run as a kernel (no 4 GiB limit on the total extent); copies touching host
+ * memory go through the HIP runtime.
+ * Replaces: HipFFTBackend::memcpy2d (fft_backend_hipfft.hpp:496-501).
+ */
+int pfx_memcpy2d_async(void *dst, size_t dpitch, const void *src, size_t spitch,
+                       size_t width, size_t height, pfx_stream stream);
+
+/* Replaces: the HIP stream owned by HipFFTBackend (fft_backend_hipfft.hpp:206, 231-235, 506-508). */
+int pfx_stream_create(pfx_stream *stream);
+int pfx_stream_destroy(pfx_stream stream);
+int pfx_stream_synchronize(pfx_stream stream);
+
+/* ------------------------------------------------------------ pack/unpack */
+
+/*
+ * Pack for every peer p: src_leading rows of src_n[p]*src_trailing elements
+ * of elem_size bytes, read from src + src_s[p]*src_trailing (row pitch
+ * src_extent*src_trailing elements) and written contiguously at
+ * pack + send_displs[p].  Bit-exact C-order subarray serialisation.  One
+ * launch for all peers.  Arrays are host arrays of nparts entries.
+ * Replaces: the pack loops of exchange_packed / exchange_hybrid
+ *   (parafaft_common.hpp:576-590, 711-730).
+ */
+int pfx_pack(void *pack, const void *src, int nparts, const int *src_n, const int *src_s,
+             const uint64_t *send_displs, uint64_t src_leading, uint64_t src_trailing,
+             uint64_t src_extent, uint64_t elem_size, pfx_stream stream);
+
+/*
+ * Unpack, the mirror of pfx_pack: for every peer p, dst_leading rows of
+ * dst_n[p]*dst_trailing elements from recv + recv_displs[p] into
+ * dst + dst_s[p]*dst_trailing with row pitch dst_extent*dst_trailing.
+ * Replaces: the unpack loops (parafaft_common.hpp:654-668, 844-857).
+ */
+int pfx_unpack(void *dst, const void *recv, int nparts, const int *dst_n, const int *dst_s,
+               const uint64_t *recv_displs, uint64_t dst_leading, uint64_t dst_trailing,
+               uint64_t dst_extent, uint64_t elem_size, pfx_stream stream);
+
+/* ------------------------------------------------------------------ scale */
+
+/*
+ * data[i] *= s for `count` scalars of the given precision (PFX_F32 / PFX_F64).
+ * Replaces: the user-side normalisation loop after backward
+ *   (test/c2c/test_fftw_c2c_roundtrip_nd.cpp:73-78, example/c2c.cpp:27-31).
+ */
+int pfx_scale(void *data, int64_t count, int precision, double s, pfx_stream stream);
+
+/* --------------------------------------------- one-rank N-D fast path */
+
+/*
+ * Whole-array in-place transform of a row-major array of `ndim` axes on one
+ * GPU: one launch per axis, no scratch copies.  kind PFX_C2C: complex array of
+ * `shape`.  kind PFX_R2C: padded real array, last axis stored as
+ * 2*(shape[ndim-1]/2+1) scalars; forward = R2C, backward = C2R.
+ * Replaces, on one rank: ParaFaFT_C2C::forward/backward (parafaft_c2c.hpp:539-694)
+ *   and ParaFaFT_R2C::forward_in_place/backward_in_place (parafaft_r2c.hpp:360-542)
+ *   without the exchange_local copies (parafaft_common.hpp:528-535).
+ */
+int pfx_ndplan_create(pfx_ndplan *plan, int precision, int kind, int ndim, const int *shape);
+
+/* scale: multiplies the result inside the last pass (1.0 = unnormalised). */
+int pfx_ndplan_execute(pfx_ndplan plan, int direction, void *data, double scale,
+                       pfx_stream stream);
+int pfx_ndplan_destroy(pfx_ndplan plan);
+
+/* Stage plan of axis-stage s (0 = last axis), for per-stage timing. */
+int pfx_ndplan_stage(pfx_ndplan plan, int stage, pfx_plan *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFX_CAPI_H */

@@ -1,0 +1,523 @@
+// pfx_capi.hip -- implementation of the C ABI declared in include/pfx_capi.h.
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/pfx_capi.h"
+#include "pfx_internal.hpp"
+
+using namespace pfx;
+
+namespace {
+
+thread_local std::string g_last_error;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *what) {
+  return fail(PFX_ERR_HIP, "HIP error in %s: %s", what, hipGetErrorString(e));
+}
+
+#define PFX_HIP(call, what)                    \
+  do {                                         \
+    hipError_t e_ = (call);                    \
+    if (e_ != hipSuccess) return hip_fail(e_, what); \
+  } while (0)
+
+struct Registry {
+  std::vector<const KernelEntry *> entries;
+  Registry() {
+    const KernelEntry *(*fns[])(int *) = {table_f32_row, table_f32_col, table_f32_real,
+                                          table_f64_row, table_f64_col, table_f64_real};
+    for (auto fn : fns) {
+      int n = 0;
+      const KernelEntry *t = fn(&n);
+      for (int i = 0; i < n; ++i) entries.push_back(&t[i]);
+    }
+  }
+};
+
+const Registry &registry() {
+  static Registry r;
+  return r;
+}
+
+// Environment override of the batch-contiguous tile: PFX_COL_TILE=<per_block>.
+int col_tile_override() {
+  static int v = [] {
+    const char *s = getenv("PFX_COL_TILE");
+    return s ? atoi(s) : 0;
+  }();
+  return v;
+}
+
+const KernelEntry *find_kernel(int prec, int kind, int64_t n, bool col) {
+  const KernelEntry *best = nullptr;
+  const int want = col ? col_tile_override() : 0;
+  for (const KernelEntry *e : registry().entries) {
+    if (e->prec != prec || e->kind != kind || e->n != n) continue;
+    if (kind == KIND_C2C && e->col != (col ? 1 : 0)) continue;
+    if (want > 0) {
+      if (e->per_block == want) return e;
+      if (!best && e->preferred) best = e;
+      continue;
+    }
+    if (e->preferred) return e;
+    if (!best) best = e;
+  }
+  return best;
+}
+
+int base_prec(int precision) {
+  return precision == PFX_F64 ? 1 : 0;
+}
+
+}  // namespace
+
+struct pfx_plan_t {
+  int precision = PFX_F32;
+  int kind = PFX_C2C;
+  int64_t length = 0;     // transform length (real length for R2C/C2R)
+  int64_t inner = 0, outer = 1;
+  int64_t stride = 1, dist = 0, dist_outer = 0;
+  const KernelEntry *kernel = nullptr;  // null: direct-DFT fallback
+  void *tw = nullptr;
+  int64_t tw_len = 0;
+};
+
+struct pfx_ndplan_t {
+  int precision = PFX_F32;
+  int kind = PFX_C2C;
+  int ndim = 0;
+  std::vector<int> shape;
+  std::vector<pfx_plan> stages;  // stages[s] transforms axis ndim-1-s
+};
+
+extern "C" {
+
+const char *pfx_error_string(int status) {
+  switch (status) {
+    case PFX_SUCCESS: return "success";
+    case PFX_ERR_INVALID_ARG: return "invalid argument";
+    case PFX_ERR_UNSUPPORTED: return "unsupported configuration";
+    case PFX_ERR_HIP: return "HIP runtime error";
+    case PFX_ERR_ALLOC: return "allocation failure";
+    case PFX_ERR_INTERNAL: return "internal error";
+    default: return "unknown status";
+  }
+}
+
+const char *pfx_last_error_message(void) { return g_last_error.c_str(); }
+
+const char *pfx_version(void) { return "parafaft-mi355 0.1 (gfx950)"; }
+
+static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, int64_t inner,
+                       int64_t stride, int64_t dist, int64_t outer, int64_t dist_outer) {
+  if (!out) return fail(PFX_ERR_INVALID_ARG, "plan pointer is null");
+  *out = nullptr;
+  if (precision != PFX_F32 && precision != PFX_F64)
+    return fail(PFX_ERR_UNSUPPORTED, "precision %d has no tiled kernels yet", precision);
+  if (length <= 0) return fail(PFX_ERR_INVALID_ARG, "length must be positive (got %lld)", (long long)length);
+  if (inner < 0 || outer < 0) return fail(PFX_ERR_INVALID_ARG, "batch must be non-negative");
+  if (stride <= 0) return fail(PFX_ERR_INVALID_ARG, "stride must be positive (got %lld)", (long long)stride);
+  if ((kind == PFX_R2C || kind == PFX_C2R) && stride != 1)
+    return fail(PFX_ERR_UNSUPPORTED, "R2C/C2R plans support stride 1 only (got %lld)", (long long)stride);
+  if ((kind == PFX_R2C || kind == PFX_C2R) && dist < 2 * (length / 2 + 1) && inner > 1)
+    return fail(PFX_ERR_INVALID_ARG, "padded dist %lld is below 2*(n/2+1)", (long long)dist);
+  if ((kind == PFX_R2C || kind == PFX_C2R) && (dist % 2) != 0 && inner > 1)
+    return fail(PFX_ERR_INVALID_ARG, "padded dist must be even (complex-aligned rows)");
+  const int64_t total = inner * outer;
+  if (total > (int64_t)0x7fffffff * 1)
+    ;  // the grid is sized per tile below; checked there
+
+  pfx_plan p = new (std::nothrow) pfx_plan_t;
+  if (!p) return fail(PFX_ERR_ALLOC, "out of host memory");
+  p->precision = precision;
+  p->kind = kind;
+  p->length = length;
+  p->inner = inner;
+  p->outer = outer;
+  p->stride = stride;
+  p->dist = dist;
+  p->dist_outer = dist_outer;
+
+  const int bp = base_prec(precision);
+  const size_t esz = bp ? 16 : 8;
+  int64_t lds_n = length;
+  int kk = KIND_C2C;
+  if (kind == PFX_R2C || kind == PFX_C2R) {
+    kk = kind == PFX_R2C ? KIND_R2C : KIND_C2R;
+    lds_n = (length % 2 == 0) ? length / 2 : -1;
+  }
+  // Batch-contiguous layout when consecutive transforms are adjacent.
+  const bool col = (kind == PFX_C2C) && dist == 1 && stride > 1;
+  if (lds_n > 0) p->kernel = find_kernel(bp, kk, lds_n, col);
+  if (!p->kernel) {
+    if (length > naive_max_length(bp)) {
+      delete p;
+      return fail(PFX_ERR_UNSUPPORTED,
+                  "length %lld has no tiled kernel and exceeds the direct-DFT fallback "
+                  "limit of %d", (long long)length, naive_max_length(bp));
+    }
+  }
+  if (p->kernel) {
+    const int64_t blocks = (total + p->kernel->per_block - 1) / p->kernel->per_block;
+    if (blocks > 0x7fffffff) {
+      delete p;
+      return fail(PFX_ERR_UNSUPPORTED, "batch too large for one launch");
+    }
+  } else if (total > 0x7fffffff) {
+    delete p;
+    return fail(PFX_ERR_UNSUPPORTED, "batch too large for one launch");
+  }
+  // Twiddle table: C2C W_n (n entries); R2C/C2R tiled W_{2h}=W_n (n entries);
+  // fallback: W_n.
+  p->tw_len = length;
+  hipError_t e = hipMalloc(&p->tw, (size_t)p->tw_len * esz);
+  if (e != hipSuccess) {
+    delete p;
+    return hip_fail(e, "hipMalloc twiddles");
+  }
+  e = make_twiddles(bp, p->tw_len, p->tw, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    hipFree(p->tw);
+    delete p;
+    return hip_fail(e, "twiddle generation");
+  }
+  *out = p;
+  return PFX_SUCCESS;
+}
+
+int pfx_plan_create_c2c(pfx_plan *plan, int precision, int64_t length, int64_t batch,
+                        int64_t stride, int64_t dist) {
+  return plan_create(plan, precision, PFX_C2C, length, batch, stride, dist, 1, 0);
+}
+
+int pfx_plan_create_c2c_many(pfx_plan *plan, int precision, int64_t length, int64_t inner_batch,
+                             int64_t stride, int64_t dist, int64_t outer_batch,
+                             int64_t outer_dist) {
+  return plan_create(plan, precision, PFX_C2C, length, inner_batch, stride, dist, outer_batch,
+                     outer_dist);
+}
+
+int pfx_plan_create_r2c(pfx_plan *plan, int precision, int64_t length, int64_t batch,
+                        int64_t stride, int64_t dist_scalars) {
+  return plan_create(plan, precision, PFX_R2C, length, batch, stride, dist_scalars, 1, 0);
+}
+
+int pfx_plan_create_c2r(pfx_plan *plan, int precision, int64_t length, int64_t batch,
+                        int64_t stride, int64_t dist_scalars) {
+  return plan_create(plan, precision, PFX_C2R, length, batch, stride, dist_scalars, 1, 0);
+}
+
+int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "direction must be PFX_FORWARD or PFX_BACKWARD");
+  if (p->kind == PFX_R2C && direction != PFX_FORWARD)
+    return fail(PFX_ERR_INVALID_ARG, "an R2C plan only runs forward");
+  if (p->kind == PFX_C2R && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "a C2R plan only runs backward");
+  const int64_t total = p->inner * p->outer;
+  if (total == 0) return PFX_SUCCESS;
+  if (!data) return fail(PFX_ERR_INVALID_ARG, "data is null");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const bool real = p->kind != PFX_C2C;
+  if (p->kernel) {
+    StockhamArgs a;
+    a.data = data;
+    a.tw = p->tw;
+    a.total = total;
+    a.inner = p->inner;
+    a.stride = p->stride;
+    // R2C/C2R tiled kernels address the row as complex values.
+    a.dist = real ? p->dist / 2 : p->dist;
+    a.dist_outer = real ? p->dist_outer / 2 : p->dist_outer;
+    a.scale = scale;
+    a.use_scale = scale != 1.0;
+    a.backward = direction == PFX_BACKWARD;
+    p->kernel->launch(a, s);
+  } else {
+    NaiveArgs a;
+    a.data = data;
+    a.tw = p->tw;
+    a.total = total;
+    a.inner = p->inner;
+    a.stride = p->stride;
+    a.dist = p->dist;
+    a.dist_outer = p->dist_outer;
+    a.n = (int)p->length;
+    a.kind = p->kind == PFX_C2C ? KIND_C2C : (p->kind == PFX_R2C ? KIND_R2C : KIND_C2R);
+    a.backward = direction == PFX_BACKWARD;
+    a.scale = scale;
+    a.use_scale = scale != 1.0;
+    launch_naive(base_prec(p->precision), a, s);
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return PFX_SUCCESS;
+}
+
+int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream) {
+  return pfx_execute_scaled(plan, direction, data, 1.0, stream);
+}
+
+int pfx_plan_destroy(pfx_plan p) {
+  if (!p) return PFX_SUCCESS;
+  if (p->tw) hipFree(p->tw);
+  delete p;
+  return PFX_SUCCESS;
+}
+
+int pfx_plan_describe(pfx_plan p, char *name, size_t name_len, int *per_block, int *threads) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  char buf[256];
+  if (p->kernel) {
+    const KernelEntry *k = p->kernel;
+    snprintf(buf, sizeof(buf), "%s_%s_%s_n%d_r[%s]", k->prec ? "f64" : "f32",
+             k->kind == KIND_C2C ? "c2c" : (k->kind == KIND_R2C ? "r2c" : "c2r"),
+             k->col ? "col" : "row", k->n, k->radices);
+    if (per_block) *per_block = k->per_block;
+    if (threads) *threads = k->tpt * k->per_block;
+  } else {
+    snprintf(buf, sizeof(buf), "%s_direct_dft_n%lld", p->precision == PFX_F64 ? "f64" : "f32",
+             (long long)p->length);
+    if (per_block) *per_block = 1;
+    if (threads) *threads = 256;
+  }
+  if (name && name_len) {
+    strncpy(name, buf, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  return PFX_SUCCESS;
+}
+
+// ------------------------------------------------------------------ memory
+
+int pfx_malloc(void **ptr, size_t bytes) {
+  if (!ptr) return fail(PFX_ERR_INVALID_ARG, "ptr is null");
+  *ptr = nullptr;
+  if (bytes == 0) return PFX_SUCCESS;
+  hipError_t e = hipMalloc(ptr, bytes);
+  if (e != hipSuccess) return fail(PFX_ERR_ALLOC, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+  return PFX_SUCCESS;
+}
+
+int pfx_free(void *ptr) {
+  if (!ptr) return PFX_SUCCESS;
+  PFX_HIP(hipFree(ptr), "hipFree");
+  return PFX_SUCCESS;
+}
+
+int pfx_memcpy_async(void *dst, const void *src, size_t bytes, pfx_stream stream) {
+  if (bytes == 0 || dst == src) return PFX_SUCCESS;
+  PFX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, static_cast<hipStream_t>(stream)),
+          "hipMemcpyAsync");
+  return PFX_SUCCESS;
+}
+
+static bool is_device_ptr(const void *p) {
+  hipPointerAttribute_t attr;
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice;
+}
+
+int pfx_memcpy2d_async(void *dst, size_t dpitch, const void *src, size_t spitch, size_t width,
+                       size_t height, pfx_stream stream) {
+  if (width == 0 || height == 0) return PFX_SUCCESS;
+  if (dpitch < width || spitch < width)
+    return fail(PFX_ERR_INVALID_ARG, "pitch smaller than width");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (is_device_ptr(dst) && is_device_ptr(src)) {
+    PFX_HIP(launch_copy2d(dst, dpitch, src, spitch, width, height, s), "copy2d kernel");
+  } else {
+    PFX_HIP(hipMemcpy2DAsync(dst, dpitch, src, spitch, width, height, hipMemcpyDefault, s),
+            "hipMemcpy2DAsync");
+  }
+  return PFX_SUCCESS;
+}
+
+int pfx_stream_create(pfx_stream *stream) {
+  if (!stream) return fail(PFX_ERR_INVALID_ARG, "stream is null");
+  hipStream_t s;
+  PFX_HIP(hipStreamCreate(&s), "hipStreamCreate");
+  *stream = s;
+  return PFX_SUCCESS;
+}
+
+int pfx_stream_destroy(pfx_stream stream) {
+  if (!stream) return PFX_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipStreamSynchronize(s);
+  PFX_HIP(hipStreamDestroy(s), "hipStreamDestroy");
+  return PFX_SUCCESS;
+}
+
+int pfx_stream_synchronize(pfx_stream stream) {
+  PFX_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+  return PFX_SUCCESS;
+}
+
+// ------------------------------------------------------------ pack/unpack
+
+static int pack_common(bool pack, void *dst, const void *src, int nparts, const int *n,
+                       const int *st, const uint64_t *displs, uint64_t leading, uint64_t trailing,
+                       uint64_t extent, uint64_t esz, pfx_stream stream) {
+  if (nparts <= 0) return fail(PFX_ERR_INVALID_ARG, "nparts must be positive");
+  if (!n || !st || !displs) return fail(PFX_ERR_INVALID_ARG, "geometry arrays are null");
+  if (esz == 0) return fail(PFX_ERR_INVALID_ARG, "elem_size is zero");
+  std::vector<PackSeg> segs;
+  segs.reserve(nparts);
+  for (int p = 0; p < nparts; ++p) {
+    if (n[p] < 0 || st[p] < 0 || (uint64_t)st[p] + (uint64_t)n[p] > extent)
+      return fail(PFX_ERR_INVALID_ARG, "peer %d block [%d, %d) exceeds axis extent %llu", p,
+                  st[p], st[p] + n[p], (unsigned long long)extent);
+    PackSeg sg;
+    sg.row_bytes = (uint64_t)n[p] * trailing * esz;
+    const uint64_t strided = (uint64_t)st[p] * trailing * esz;
+    const uint64_t packed = displs[p] * esz;
+    sg.dst_off = pack ? packed : strided;
+    sg.src_off = pack ? strided : packed;
+    if (sg.row_bytes) segs.push_back(sg);
+  }
+  if (segs.empty() || leading == 0) return PFX_SUCCESS;
+  const uint64_t pitch = extent * trailing * esz;
+  hipError_t e = launch_pack(dst, src, segs.data(), (int)segs.size(), leading, 0, 0,
+                             pack ? 0 : pitch, pack ? pitch : 0, static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, pack ? "pack kernel" : "unpack kernel");
+  return PFX_SUCCESS;
+}
+
+int pfx_pack(void *pack, const void *src, int nparts, const int *src_n, const int *src_s,
+             const uint64_t *send_displs, uint64_t src_leading, uint64_t src_trailing,
+             uint64_t src_extent, uint64_t elem_size, pfx_stream stream) {
+  return pack_common(true, pack, src, nparts, src_n, src_s, send_displs, src_leading,
+                     src_trailing, src_extent, elem_size, stream);
+}
+
+int pfx_unpack(void *dst, const void *recv, int nparts, const int *dst_n, const int *dst_s,
+               const uint64_t *recv_displs, uint64_t dst_leading, uint64_t dst_trailing,
+               uint64_t dst_extent, uint64_t elem_size, pfx_stream stream) {
+  return pack_common(false, dst, recv, nparts, dst_n, dst_s, recv_displs, dst_leading,
+                     dst_trailing, dst_extent, elem_size, stream);
+}
+
+int pfx_scale(void *data, int64_t count, int precision, double s, pfx_stream stream) {
+  if (count < 0) return fail(PFX_ERR_INVALID_ARG, "count must be non-negative");
+  if (precision != PFX_F32 && precision != PFX_F64)
+    return fail(PFX_ERR_INVALID_ARG, "precision must be PFX_F32 or PFX_F64");
+  if (count && !data) return fail(PFX_ERR_INVALID_ARG, "data is null");
+  PFX_HIP(launch_scale(data, count, base_prec(precision), s, static_cast<hipStream_t>(stream)),
+          "scale kernel");
+  return PFX_SUCCESS;
+}
+
+// ------------------------------------------------------ one-rank N-D path
+
+int pfx_ndplan_create(pfx_ndplan *out, int precision, int kind, int ndim, const int *shape) {
+  if (!out) return fail(PFX_ERR_INVALID_ARG, "plan pointer is null");
+  *out = nullptr;
+  if (ndim < 1 || ndim > 16 || !shape) return fail(PFX_ERR_INVALID_ARG, "ndim must be in [1, 16]");
+  if (kind != PFX_C2C && kind != PFX_R2C)
+    return fail(PFX_ERR_INVALID_ARG, "kind must be PFX_C2C or PFX_R2C");
+  for (int i = 0; i < ndim; ++i)
+    if (shape[i] <= 0) return fail(PFX_ERR_INVALID_ARG, "shape[%d] must be positive", i);
+  pfx_ndplan p = new (std::nothrow) pfx_ndplan_t;
+  if (!p) return fail(PFX_ERR_ALLOC, "out of host memory");
+  p->precision = precision;
+  p->kind = kind;
+  p->ndim = ndim;
+  p->shape.assign(shape, shape + ndim);
+  std::vector<int64_t> cs(shape, shape + ndim);  // complex shape
+  if (kind == PFX_R2C) cs[ndim - 1] = shape[ndim - 1] / 2 + 1;
+  p->stages.assign(ndim, nullptr);
+  for (int s = 0; s < ndim; ++s) {
+    const int axis = ndim - 1 - s;
+    int64_t lead = 1, trail = 1;
+    for (int i = 0; i < axis; ++i) lead *= cs[i];
+    for (int i = axis + 1; i < ndim; ++i) trail *= cs[i];
+    int rc;
+    if (axis == ndim - 1) {
+      if (kind == PFX_R2C) {
+        // forward plan created here; the C2R plan is kept as a separate stage slot below
+        rc = pfx_plan_create_r2c(&p->stages[s], precision, shape[axis], lead, 1, 2 * cs[axis]);
+      } else {
+        rc = pfx_plan_create_c2c(&p->stages[s], precision, cs[axis], lead, 1, cs[axis]);
+      }
+    } else if (axis == 0) {
+      rc = pfx_plan_create_c2c(&p->stages[s], precision, cs[axis], trail, trail, 1);
+    } else {
+      rc = pfx_plan_create_c2c_many(&p->stages[s], precision, cs[axis], trail, trail, 1, lead,
+                                    cs[axis] * trail);
+    }
+    if (rc != PFX_SUCCESS) {
+      std::string msg = g_last_error;
+      pfx_ndplan_destroy(p);
+      return fail(rc, "stage %d (axis %d): %s", s, axis, msg.c_str());
+    }
+  }
+  if (kind == PFX_R2C) {
+    pfx_plan c2r = nullptr;
+    int64_t lead = 1;
+    for (int i = 0; i < ndim - 1; ++i) lead *= cs[i];
+    int rc = pfx_plan_create_c2r(&c2r, precision, shape[ndim - 1], lead, 1, 2 * cs[ndim - 1]);
+    if (rc != PFX_SUCCESS) {
+      std::string msg = g_last_error;
+      pfx_ndplan_destroy(p);
+      return fail(rc, "C2R stage: %s", msg.c_str());
+    }
+    p->stages.push_back(c2r);
+  }
+  *out = p;
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "bad direction");
+  const int D = p->ndim;
+  for (int k = 0; k < D; ++k) {
+    const int s = direction == PFX_FORWARD ? k : D - 1 - k;
+    pfx_plan pl = p->stages[s];
+    if (p->kind == PFX_R2C && s == 0 && direction == PFX_BACKWARD) pl = p->stages[D];
+    const double sc = (k == D - 1) ? scale : 1.0;
+    int rc = pfx_execute_scaled(pl, direction, data, sc, stream);
+    if (rc != PFX_SUCCESS) return rc;
+  }
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_destroy(pfx_ndplan p) {
+  if (!p) return PFX_SUCCESS;
+  for (pfx_plan s : p->stages) pfx_plan_destroy(s);
+  delete p;
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_stage(pfx_ndplan p, int stage, pfx_plan *out) {
+  if (!p || !out) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  if (stage < 0 || stage >= (int)p->stages.size()) return fail(PFX_ERR_INVALID_ARG, "bad stage");
+  *out = p->stages[stage];
+  return PFX_SUCCESS;
+}
+
+}  // extern "C"

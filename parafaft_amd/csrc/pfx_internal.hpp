@@ -1,0 +1,85 @@
+// pfx_internal.hpp -- kernel registry shared by the instantiation units and
+// the C-ABI implementation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pfx_stockham.hpp"
+
+namespace pfx {
+
+using LaunchFn = void (*)(const StockhamArgs &, hipStream_t);
+
+struct KernelEntry {
+  int prec;       // PFX_F32 / PFX_F64
+  int kind;       // KIND_C2C / KIND_R2C / KIND_C2R
+  int n;          // complex FFT length held in LDS (R2C/C2R: half the real length)
+  int col;        // 1: batch-contiguous layout, 0: row layout
+  int tpt;        // threads per transform
+  int per_block;  // transforms per workgroup
+  int lds_bytes;
+  int preferred;  // default choice among entries with the same key
+  const char *radices;
+  LaunchFn launch;
+};
+
+template <int... Rs> constexpr int prod() { return (Rs * ... * 1); }
+
+template <typename T, int TPT, int C, bool COL, int... Rs>
+void launch_c2c(const StockhamArgs &a, hipStream_t s) {
+  const int64_t nb = (a.total + C - 1) / C;
+  c2c_kernel<T, TPT, C, COL, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+}
+template <typename T, int TPT, int C, int... Rs>
+void launch_r2c(const StockhamArgs &a, hipStream_t s) {
+  const int64_t nb = (a.total + C - 1) / C;
+  r2c_kernel<T, TPT, C, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+}
+template <typename T, int TPT, int C, int... Rs>
+void launch_c2r(const StockhamArgs &a, hipStream_t s) {
+  const int64_t nb = (a.total + C - 1) / C;
+  c2r_kernel<T, TPT, C, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+}
+
+template <typename T> constexpr int prec_of();
+template <> constexpr int prec_of<float>() { return 0; }
+template <> constexpr int prec_of<double>() { return 1; }
+
+// Registry tables, one per instantiation unit.
+const KernelEntry *table_f32_row(int *count);
+const KernelEntry *table_f32_col(int *count);
+const KernelEntry *table_f32_real(int *count);
+const KernelEntry *table_f64_row(int *count);
+const KernelEntry *table_f64_col(int *count);
+const KernelEntry *table_f64_real(int *count);
+
+// Fallback kernels (any length that fits LDS): direct O(n^2) DFT.
+struct NaiveArgs {
+  void *data;
+  const void *tw;   // W_n^i, i < n (forward sign)
+  int64_t total, inner, stride, dist, dist_outer;
+  int n;
+  int kind;         // KIND_*
+  int backward;
+  int use_scale;
+  double scale;
+};
+void launch_naive(int prec, const NaiveArgs &a, hipStream_t s);
+int naive_max_length(int prec);
+
+// Twiddle table generation on the device: W_L^i = exp(-2 pi i i / L), i < L,
+// computed with sincospi in double and rounded to the table precision.
+hipError_t make_twiddles(int prec, int64_t L, void *dst, hipStream_t s);
+
+// Copy kernels.
+struct PackSeg {
+  uint64_t dst_off, src_off;  // bytes
+  uint64_t row_bytes;
+};
+hipError_t launch_pack(void *dst, const void *src, const PackSeg *segs, int nseg,
+                       uint64_t rows, uint64_t dst_pitch_mode, uint64_t src_pitch_mode,
+                       uint64_t dst_pitch, uint64_t src_pitch, hipStream_t s);
+hipError_t launch_copy2d(void *dst, size_t dpitch, const void *src, size_t spitch,
+                         size_t width, size_t height, hipStream_t s);
+hipError_t launch_scale(void *data, int64_t count, int prec, double s, hipStream_t st);
+
+}  // namespace pfx

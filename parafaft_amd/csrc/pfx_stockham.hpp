@@ -1,0 +1,304 @@
+// pfx_stockham.hpp -- batched, strided 1D FFT kernels for gfx950 (CDNA4).
+//
+// One workgroup transforms C independent 1D sequences of length N that are
+// fully resident in LDS.  The transform is a mixed-radix Stockham (autosort)
+// decimation-in-time: pass p with radix R and span Ns (product of the earlier
+// radices) maps butterfly jj in [0, N/R) to inputs jj + m N/R and outputs
+// (jj / Ns) Ns R + (jj % Ns) + m Ns, after multiplying input m by
+// W_{Ns R}^{(jj % Ns) m}.  The first pass reads straight from HBM and the last
+// pass writes straight to HBM; passes in between exchange through LDS.
+//
+// Two thread layouts cover the strided axes of a pencil:
+//   ROW  (stride == 1): the T threads of one transform are adjacent lanes, so
+//        a first-pass load of element j + m N/R is a contiguous run per wave.
+//   COL  (dist == 1, the batch index is the contiguous one): the C transforms
+//        of a workgroup are adjacent lanes, so every load/store of one element
+//        index touches C consecutive complex values -- coalesced across the
+//        batch, which is how the axis-0 and middle-axis passes of a row-major
+//        pencil reach HBM bandwidth.
+// Addresses are general: element e of transform b lives at
+//   (b / inner) * dist_outer + (b % inner) * dist + e * stride
+// so the middle axis of an N-D array (the reference's loop of leading_size
+// execute_stage calls) runs as a single launch.
+//
+// Kinds: C2C (both directions), R2C (forward, half-length complex FFT + fused
+// post-twiddle), C2R (backward, fused pre-twiddle + half-length inverse FFT).
+// An optional scale is fused into the final HBM store.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pfx_butterfly.hpp"
+
+namespace pfx {
+
+enum Kind { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
+
+struct StockhamArgs {
+  void* data;            // complex view of the buffer (R2C/C2R: padded rows)
+  const void* tw;        // W_L^i, i in [0, L): L = N (C2C) or 2N (R2C/C2R)
+  int64_t total;         // number of transforms = inner * outer
+  int64_t inner;         // transforms per outer slice
+  int64_t stride;        // element stride inside a transform (complex units)
+  int64_t dist;          // distance between transforms inside a slice
+  int64_t dist_outer;    // distance between outer slices
+  double scale;          // multiplies every output (1.0 = none)
+  int backward;          // C2C only: 1 = exp(+i...), via re/im swap
+  int use_scale;
+};
+
+template <typename T, int N, int C, bool COL> struct LdsLayout {
+  // ROW: one padded row of N per transform; one pad slot every 16 entries
+  // keeps the first pass's stride-R writes off a single bank.
+  static constexpr int NP = N + N / 16 + 1;
+  static constexpr int size = COL ? N * C : NP * C;
+  __device__ static inline int at(int idx, int c) {
+    if constexpr (COL) return idx * C + c;
+    else return c * NP + idx + (idx >> 4);
+  }
+};
+
+// One Stockham pass executed by one thread for its Q butterflies.
+// SRC/DST: 0 = LDS, 1 = HBM.
+template <typename T, int N, int R, int Ns, int TPT, int TWS, class LoadF, class StoreF>
+__device__ inline void stockham_pass(int j, const V2<T>* __restrict__ tw, LoadF load, StoreF store) {
+  constexpr int NB = N / R;                  // butterflies in this pass
+  constexpr int Q = (NB + TPT - 1) / TPT;    // butterflies per thread
+  constexpr bool EXACT = (NB % TPT) == 0;
+  V2<T> v[Q][R];
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int jj = j + q * TPT;
+    if (EXACT || jj < NB) {
+#pragma unroll
+      for (int m = 0; m < R; ++m) v[q][m] = load(jj + m * NB);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int jj = j + q * TPT;
+    if (EXACT || jj < NB) {
+      if constexpr (Ns > 1) {
+        const int kk = jj % Ns;
+        constexpr int TSTEP = (N / (Ns * R)) * TWS;
+#pragma unroll
+        for (int m = 1; m < R; ++m) v[q][m] = cmul<T>(v[q][m], tw[kk * m * TSTEP]);
+      }
+      bfly<T, R>(v[q]);
+    }
+  }
+  store.before();
+#pragma unroll
+  for (int q = 0; q < Q; ++q) {
+    const int jj = j + q * TPT;
+    if (EXACT || jj < NB) {
+      const int kk = jj % Ns;
+      const int ob = (jj / Ns) * Ns * R + kk;
+#pragma unroll
+      for (int m = 0; m < R; ++m) store(ob + m * Ns, v[q][m]);
+    }
+  }
+  store.after();
+}
+
+template <typename T> struct GlobalPtr {
+  V2<T>* p;
+  int64_t stride;
+  bool swap_io;
+  bool valid;
+  T scale;
+  bool use_scale;
+};
+
+// Run passes PI.. of the radix list.  The first pass may read HBM (FIRST_G),
+// the last may write HBM (LAST_G); everything else goes through LDS.
+template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G,
+          int PI, int Ns, int... Rs>
+struct Passes;
+
+template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
+          int Ns>
+struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns> {
+  __device__ static inline void run(int, int, V2<T>*, const V2<T>*, const GlobalPtr<T>&) {}
+};
+
+template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
+          int Ns, int R, int... Rest>
+struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
+  using L = LdsLayout<T, N, C, COL>;
+  __device__ static inline void run(int j, int c, V2<T>* lds, const V2<T>* tw, const GlobalPtr<T>& g) {
+    constexpr bool IS_FIRST = (PI == 0);
+    constexpr bool IS_LAST = (sizeof...(Rest) == 0);
+    auto load = [&](int e) -> V2<T> {
+      if constexpr (IS_FIRST && FIRST_G) {
+        V2<T> x = mk<T>(T(0), T(0));
+        if (g.valid) x = g.p[e * g.stride];
+        return g.swap_io ? swp<T>(x) : x;
+      } else {
+        return lds[L::at(e, c)];
+      }
+    };
+    struct StoreLds {
+      V2<T>* lds; int c;
+      __device__ void before() const { __syncthreads(); }
+      __device__ void after() const { __syncthreads(); }
+      __device__ void operator()(int e, V2<T> v) const { lds[L::at(e, c)] = v; }
+    };
+    struct StoreG {
+      const GlobalPtr<T>* g;
+      // A single pass that both reads and writes HBM in place must see every
+      // lane's loads done before any lane stores (only matters when TPT > 1).
+      __device__ void before() const {
+        if constexpr (IS_FIRST && FIRST_G && TPT > 1) __syncthreads();
+      }
+      __device__ void after() const {}
+      __device__ void operator()(int e, V2<T> v) const {
+        if (g->use_scale) v = scl<T>(v, g->scale);
+        if (g->swap_io) v = swp<T>(v);
+        if (g->valid) g->p[e * g->stride] = v;
+      }
+    };
+    if constexpr (IS_LAST && LAST_G) {
+      stockham_pass<T, N, R, Ns, TPT, TWS>(j, tw, load, StoreG{&g});
+    } else {
+      stockham_pass<T, N, R, Ns, TPT, TWS>(j, tw, load, StoreLds{lds, c});
+    }
+    Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI + 1, Ns * R, Rest...>::run(j, c, lds, tw, g);
+  }
+};
+
+template <int TPT, int C, bool COL>
+__device__ inline void thread_coords(int& j, int& c) {
+  const int tid = threadIdx.x;
+  if constexpr (COL) { c = tid % C; j = tid / C; }
+  else { j = tid % TPT; c = tid / TPT; }
+}
+
+__device__ inline int64_t xform_offset(int64_t b, const StockhamArgs& a) {
+  const int64_t o = b / a.inner;
+  const int64_t i = b - o * a.inner;
+  return o * a.dist_outer + i * a.dist;
+}
+
+// ---------------------------------------------------------------------------
+// C2C
+// ---------------------------------------------------------------------------
+template <typename T, int TPT, int C, bool COL, int... Rs>
+__global__ void __launch_bounds__(TPT * C)
+c2c_kernel(StockhamArgs a) {
+  constexpr int N = (Rs * ... * 1);
+  using L = LdsLayout<T, N, C, COL>;
+  __shared__ V2<T> lds[L::size];
+  int j, c;
+  thread_coords<TPT, C, COL>(j, c);
+  const int64_t b = (int64_t)blockIdx.x * C + c;
+  GlobalPtr<T> g;
+  g.valid = b < a.total;
+  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  g.stride = a.stride;
+  g.swap_io = a.backward != 0;
+  g.scale = T(a.scale);
+  g.use_scale = a.use_scale != 0;
+  const V2<T>* tw = static_cast<const V2<T>*>(a.tw);
+  Passes<T, N, TPT, C, COL, 1, true, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
+}
+
+// ---------------------------------------------------------------------------
+// R2C (forward): row of 2N reals = N complex z[n] = x[2n] + i x[2n+1];
+// Z = FFT_N(z); X[k] = (Z[k] + conj Z[N-k])/2 - (i/2) W_2N^k (Z[k] - conj Z[N-k]),
+// k = 0..N, written as N+1 complex values over the same padded row.
+// Row layout only (stride 1).  a.dist is in complex units (= padded dist / 2).
+// ---------------------------------------------------------------------------
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C)
+r2c_kernel(StockhamArgs a) {
+  constexpr int N = (Rs * ... * 1);
+  using L = LdsLayout<T, N, C, false>;
+  __shared__ V2<T> lds[L::size];
+  int j, c;
+  thread_coords<TPT, C, false>(j, c);
+  const int64_t b = (int64_t)blockIdx.x * C + c;
+  GlobalPtr<T> g;
+  g.valid = b < a.total;
+  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  g.stride = 1;
+  g.swap_io = false;
+  g.scale = T(a.scale);
+  g.use_scale = a.use_scale != 0;
+  const V2<T>* tw = static_cast<const V2<T>*>(a.tw);  // W_{2N}^i, i < 2N
+  Passes<T, N, TPT, C, false, 2, true, false, 0, 1, Rs...>::run(j, c, lds, tw, g);
+  // post-twiddle: pairs (k, N-k) for k = 0..N/2
+  const T half = T(0.5);
+  for (int k = j; k <= N / 2; k += TPT) {
+    const int km = (N - k) % N;
+    V2<T> A = lds[L::at(k, c)];
+    V2<T> Bm = lds[L::at(km, c)];
+    // X[k]
+    {
+      V2<T> B = conj<T>(Bm);
+      V2<T> s = add<T>(A, B), d = sub<T>(A, B);
+      V2<T> wd = cmul<T>(tw[k], d);
+      V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
+      if (g.use_scale) x = scl<T>(x, g.scale);
+      if (g.valid) g.p[k] = x;
+    }
+    // X[N-k] unless k is its own mirror (2k == N); k = 0 gives X[N]
+    if (2 * k != N) {
+      const int kk = N - k;
+      V2<T> A2 = Bm, B2 = conj<T>(A);
+      V2<T> s = add<T>(A2, B2), d = sub<T>(A2, B2);
+      V2<T> wd = cmul<T>(tw[kk], d);
+      V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
+      if (g.use_scale) x = scl<T>(x, g.scale);
+      if (g.valid) g.p[kk] = x;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// C2R (backward): X[0..N] -> Z[k] = (X[k] + conj X[N-k]) + i W_2N^-k (X[k] - conj X[N-k]),
+// Im X[0] and Im X[N] ignored (Hermitian input); z = IFFT_N(Z) (unnormalised)
+// gives z[n] = x[2n] + i x[2n+1], stored as the N complex slots of the row.
+// ---------------------------------------------------------------------------
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C)
+c2r_kernel(StockhamArgs a) {
+  constexpr int N = (Rs * ... * 1);
+  using L = LdsLayout<T, N, C, false>;
+  __shared__ V2<T> lds[L::size];
+  int j, c;
+  thread_coords<TPT, C, false>(j, c);
+  const int64_t b = (int64_t)blockIdx.x * C + c;
+  GlobalPtr<T> g;
+  g.valid = b < a.total;
+  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  g.stride = 1;
+  g.swap_io = true;  // inverse FFT via re/im swap; pre-twiddled data is stored swapped
+  g.scale = T(a.scale);
+  g.use_scale = a.use_scale != 0;
+  const V2<T>* tw = static_cast<const V2<T>*>(a.tw);
+  for (int k = j; k <= N / 2; k += TPT) {
+    const int km = N - k;
+    V2<T> Xk = mk<T>(T(0), T(0)), Xm = Xk;
+    if (g.valid) { Xk = g.p[k]; Xm = g.p[km]; }
+    if (k == 0) { Xk.y = T(0); Xm.y = T(0); }
+    // Z[k]
+    {
+      V2<T> B = conj<T>(Xm);
+      V2<T> s = add<T>(Xk, B), d = sub<T>(Xk, B);
+      V2<T> wd = cmulc<T>(d, tw[k]);        // W^-k d
+      V2<T> z = mk<T>(s.x - wd.y, s.y + wd.x);  // s + i wd
+      lds[L::at(k, c)] = swp<T>(z);
+    }
+    if (k != 0 && 2 * k != N) {
+      V2<T> B = conj<T>(Xk);
+      V2<T> s = add<T>(Xm, B), d = sub<T>(Xm, B);
+      V2<T> wd = cmulc<T>(d, tw[km]);
+      V2<T> z = mk<T>(s.x - wd.y, s.y + wd.x);
+      lds[L::at(km, c)] = swp<T>(z);
+    }
+  }
+  __syncthreads();
+  Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
+}
+
+}  // namespace pfx
