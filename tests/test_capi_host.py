@@ -1,0 +1,97 @@
+"""CPU checks of the C ABI: libpfx.so loads, exports every symbol that
+include/pfx_capi.h declares, and validates arguments on the host (no device
+work is launched here)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared_symbols():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if not fn.endswith(".h"):
+            continue
+        text = open(os.path.join(ROOT, "include", fn)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names.update(re.findall(r"\b(pfx_[a-z0-9_]+)\s*\(", text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    import parafaft_amd as P
+    from parafaft_amd import _capi as A
+    L = P.load_library()
+    declared = _declared_symbols()
+    assert len(declared) >= 25
+    for name in declared:
+        assert hasattr(L, name), name
+    bound = {n for n, _, _ in A.SIGNATURES}
+    assert declared == bound, declared ^ bound
+
+
+def test_error_strings_and_version():
+    import parafaft_amd as P
+    L = P.load_library()
+    assert L.pfx_error_string(0) == b"success"
+    assert L.pfx_error_string(2) == b"unsupported configuration"
+    assert b"gfx950" in L.pfx_version()
+
+
+@pytest.mark.parametrize("args,code", [
+    ((0, 0, 4, 1, 0), 1),        # length 0
+    ((0, 16, 4, 0, 16), 1),      # stride 0
+    ((7, 16, 4, 1, 16), 2),      # unknown precision
+    ((0, 16, -1, 1, 16), 1),     # negative batch
+])
+def test_plan_argument_validation(args, code):
+    import parafaft_amd as P
+    L = P.load_library()
+    h = C.c_void_p()
+    prec, n, batch, stride, dist = args
+    rc = L.pfx_plan_create_c2c(C.byref(h), prec, n, batch, stride, dist)
+    assert rc == code
+    assert h.value is None
+    assert L.pfx_last_error_message()
+
+
+def test_r2c_validation():
+    import parafaft_amd as P
+    L = P.load_library()
+    h = C.c_void_p()
+    assert L.pfx_plan_create_r2c(C.byref(h), 0, 16, 4, 2, 18) == 2      # stride != 1
+    assert L.pfx_plan_create_r2c(C.byref(h), 0, 16, 4, 1, 16) == 1      # dist < 2(n/2+1)
+    assert L.pfx_plan_create_c2r(C.byref(h), 0, 16, 4, 1, 17) == 1      # odd dist
+
+
+def test_python_mirror_raises_pfx_error():
+    import parafaft_amd as P
+    with pytest.raises(P.PfxError) as e:
+        P.Plan.c2c("f32", 0, 1, 1, 1)
+    assert "length" in str(e.value)
+
+
+def test_execute_and_pack_reject_bad_input_without_device_work():
+    import parafaft_amd as P
+    L = P.load_library()
+    assert L.pfx_execute(None, 0, None, None) == 1
+    n = (C.c_int * 2)(3, 4)
+    s = (C.c_int * 2)(0, 3)
+    d = (C.c_uint64 * 2)(0, 3)
+    # block [3, 7) exceeds an axis extent of 6
+    assert L.pfx_pack(None, None, 2, n, s, d, 1, 1, 6, 8, None) == 1
+    assert L.pfx_scale(None, 5, 0, 2.0, None) == 1
+    assert L.pfx_ndplan_create(C.byref(C.c_void_p()), 0, 1, 0, None) == 1
+
+
+def test_no_cpu_fallback_in_product():
+    """The shipped package never imports the oracle."""
+    pkg = os.path.join(ROOT, "parafaft_amd")
+    for dirpath, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith(".py"):
+                src = open(os.path.join(dirpath, f)).read()
+                assert "oracle" not in src.replace("ORACLE", ""), f
