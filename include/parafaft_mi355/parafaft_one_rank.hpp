@@ -1,0 +1,330 @@
+/**
+ * @file parafaft_one_rank.hpp
+ * @brief One-rank ParaFaFT drivers for any backend of the concept.
+ *
+ * parafaft_mi355::ParaFaFT_C2C<D, Backend> and ParaFaFT_R2C<D, Backend> restate,
+ * for a single rank and without MPI, the stage loops of the reference drivers:
+ * plan geometry (parafaft_c2c.hpp:249-287, parafaft_r2c.hpp:1158-1206), the
+ * ping-pong parity and per-stage loop (parafaft_c2c.hpp:539-694,
+ * parafaft_r2c.hpp:360-542), perform_fft_on_buffer with its per-slice loop on
+ * middle axes (parafaft_c2c.hpp:866-897, parafaft_r2c.hpp:1219-1245), and the
+ * one-rank exchange, which is an identity flat copy (exchange_local,
+ * parafaft_common.hpp:467-543).  Accessor names and meaning follow
+ * parafaft_c2c.hpp:348-426 and parafaft_r2c.hpp:552-796.
+ *
+ * With Mode::Fast the driver runs the pfx one-rank N-D plan instead: one launch
+ * per axis, no exchange copies, no per-slice launches (the results are the
+ * same: on one rank every exchange is the identity).
+ */
+#ifndef PARAFAFT_MI355_ONE_RANK_HPP
+#define PARAFAFT_MI355_ONE_RANK_HPP
+
+#include <cstddef>
+#include <utility>
+
+#include "fft_backend_mi355.hpp"
+
+namespace parafaft_mi355 {
+
+using parafaft::FFTDirection;
+using parafaft::FFTPlanFlag;
+
+enum class Mode { Faithful, Fast };
+
+template <typename B> constexpr int nd_precision() {
+  return std::is_same<typename B::FloatType, double>::value ? PFX_F64 : PFX_F32;
+}
+
+template <int D, typename Backend = parafaft::MI355Backend<double>>
+class ParaFaFT_C2C {
+  static_assert(D >= 1, "D >= 1");
+
+public:
+  using FloatType = typename Backend::FloatType;
+  using Complex = typename Backend::Complex;
+  using ComplexBuffer = typename Backend::ComplexBuffer;
+
+  ParaFaFT_C2C(const int global_shape[D], FFTPlanFlag flag = FFTPlanFlag::Estimate,
+               Mode mode = Mode::Faithful)
+      : backend_(D, flag), mode_(mode) {
+    for (int i = 0; i < D; ++i) global_shape_[i] = global_shape[i];
+    size_ = 1;
+    for (int i = 0; i < D; ++i) size_ *= static_cast<std::size_t>(global_shape_[i]);
+    if (mode_ == Mode::Fast) {
+      parafaft::check_pfx(pfx_ndplan_create(&nd_, nd_precision<Backend>(), PFX_C2C, D,
+                                            global_shape_),
+                          "pfx_ndplan_create");
+      return;
+    }
+    scratch_.resize(size_);
+    create_backend_plans();
+  }
+
+  ~ParaFaFT_C2C() { pfx_ndplan_destroy(nd_); }
+  ParaFaFT_C2C(const ParaFaFT_C2C &) = delete;
+  ParaFaFT_C2C &operator=(const ParaFaFT_C2C &) = delete;
+
+  std::size_t get_local_size() const { return size_; }
+  std::size_t get_required_output_size() const { return size_; }
+  void get_local_shape(int shape[D]) const { copy(shape, global_shape_); }
+  void get_global_start(int start[D]) const { zero(start); }
+  void get_final_shape(int shape[D]) const { copy(shape, global_shape_); }
+  void get_final_start(int start[D]) const { zero(start); }
+  void get_domain_decomposition(int dec[D]) const {
+    for (int i = 0; i < D; ++i) dec[i] = 1;
+  }
+
+  /// parafaft_c2c.hpp:539-602 (unnormalised).
+  void forward(Complex *data) { run(data, true, 1.0); }
+
+  /// parafaft_c2c.hpp:632-694.  scale != 1 is fused into the last pass
+  /// (the reference leaves 1/N to the user).
+  void backward(Complex *data, double scale = 1.0) { run(data, false, scale); }
+
+  Backend &backend() { return backend_; }
+
+private:
+  static void copy(int *dst, const int *src) {
+    for (int i = 0; i < D; ++i) dst[i] = src[i];
+  }
+  static void zero(int *dst) {
+    for (int i = 0; i < D; ++i) dst[i] = 0;
+  }
+  std::size_t prod(int from, int to) const {
+    std::size_t p = 1;
+    for (int i = from; i < to; ++i) p *= static_cast<std::size_t>(global_shape_[i]);
+    return p;
+  }
+
+  void create_backend_plans() {
+    for (int stage = 0; stage < D; ++stage) {
+      const int axis = D - 1 - stage;
+      const int length = global_shape_[axis];
+      if (axis == D - 1) {
+        backend_.create_stage_plan(stage, length, prod(0, D - 1), scratch_.data(), 1, length);
+      } else if (axis == 0) {
+        const std::size_t batch = prod(1, D);
+        backend_.create_stage_plan(stage, length, batch, scratch_.data(),
+                                   static_cast<std::ptrdiff_t>(batch), 1);
+      } else {
+        const std::size_t trailing = prod(axis + 1, D);
+        backend_.create_stage_plan(stage, length, trailing, scratch_.data(),
+                                   static_cast<std::ptrdiff_t>(trailing), 1);
+      }
+    }
+  }
+
+  void perform_fft_on_buffer(int stage, int axis, FFTDirection dir, Complex *buf, double scale) {
+    auto exec = [&](Complex *p) {
+      if (scale == 1.0) backend_.execute_stage(stage, dir, p);
+      else backend_.execute_stage_scaled(stage, dir, p, scale);
+    };
+    if (axis == D - 1 || axis == 0) {
+      exec(buf);
+    } else {
+      const std::size_t leading = prod(0, axis);
+      const std::size_t slice = static_cast<std::size_t>(global_shape_[axis]) * prod(axis + 1, D);
+      for (std::size_t i = 0; i < leading; ++i) exec(buf + i * slice);
+    }
+  }
+
+  void run(Complex *data, bool forward, double scale) {
+    if (mode_ == Mode::Fast) {
+      parafaft::check_pfx(pfx_ndplan_execute(nd_, forward ? PFX_FORWARD : PFX_BACKWARD, data,
+                                             scale, backend_.stream()),
+                          "pfx_ndplan_execute");
+      backend_.sync();
+      return;
+    }
+    const std::size_t bytes = size_ * sizeof(Complex);
+    Complex *src, *dst;
+    if ((D - 1) % 2 == 0) {
+      src = data;
+      dst = scratch_.data();
+    } else {
+      backend_.memcpy(scratch_.data(), data, bytes);
+      src = scratch_.data();
+      dst = data;
+    }
+    const FFTDirection dir = forward ? FFTDirection::Forward : FFTDirection::Backward;
+    for (int k = 0; k < D; ++k) {
+      const int stage = forward ? k : D - 1 - k;
+      const int axis = D - 1 - stage;
+      perform_fft_on_buffer(stage, axis, dir, src, k == D - 1 ? scale : 1.0);
+      if (k < D - 1) {
+        backend_.memcpy(dst, src, bytes);  // one-rank exchange_local: identity
+        std::swap(src, dst);
+      }
+    }
+    backend_.sync();
+  }
+
+  Backend backend_;
+  Mode mode_;
+  int global_shape_[D];
+  std::size_t size_ = 0;
+  ComplexBuffer scratch_;
+  pfx_ndplan nd_ = nullptr;
+};
+
+template <int D, typename Backend = parafaft::MI355Backend<double>>
+class ParaFaFT_R2C {
+public:
+  using FloatType = typename Backend::FloatType;
+  using Complex = typename Backend::Complex;
+  using ComplexBuffer = typename Backend::ComplexBuffer;
+
+  ParaFaFT_R2C(const int global_shape[D], FFTPlanFlag flag = FFTPlanFlag::Estimate,
+               Mode mode = Mode::Faithful)
+      : backend_(D, flag), mode_(mode) {
+    for (int i = 0; i < D; ++i) real_[i] = cplx_[i] = global_shape[i];
+    cplx_[D - 1] = real_[D - 1] / 2 + 1;
+    csize_ = 1;
+    for (int i = 0; i < D; ++i) csize_ *= static_cast<std::size_t>(cplx_[i]);
+    if (mode_ == Mode::Fast) {
+      parafaft::check_pfx(pfx_ndplan_create(&nd_, nd_precision<Backend>(), PFX_R2C, D, real_),
+                          "pfx_ndplan_create");
+      return;
+    }
+    scratch_.resize(csize_);
+    create_backend_plans();
+  }
+
+  ~ParaFaFT_R2C() { pfx_ndplan_destroy(nd_); }
+  ParaFaFT_R2C(const ParaFaFT_R2C &) = delete;
+  ParaFaFT_R2C &operator=(const ParaFaFT_R2C &) = delete;
+
+  /// parafaft_r2c.hpp:573-596, in FloatType scalars.
+  std::size_t get_required_output_size() const {
+    const std::size_t stage0 = cprod(0, D - 1) * 2 * static_cast<std::size_t>(cplx_[D - 1]);
+    return stage0 > 2 * csize_ ? stage0 : 2 * csize_;
+  }
+  std::size_t get_local_real_size() const { return rprod(0, D); }
+  std::size_t get_local_complex_size() const { return csize_; }
+  void get_local_real_shape(int s[D]) const {
+    for (int i = 0; i < D; ++i) s[i] = real_[i];
+  }
+  void get_local_complex_shape(int s[D]) const {
+    for (int i = 0; i < D; ++i) s[i] = cplx_[i];
+  }
+  void get_real_global_start(int s[D]) const {
+    for (int i = 0; i < D; ++i) s[i] = 0;
+  }
+  void get_complex_global_start(int s[D]) const {
+    for (int i = 0; i < D; ++i) s[i] = 0;
+  }
+
+  /// parafaft_r2c.hpp:360-430.
+  void forward_in_place(FloatType *padded) {
+    if (mode_ == Mode::Fast) {
+      parafaft::check_pfx(pfx_ndplan_execute(nd_, PFX_FORWARD, padded, 1.0, backend_.stream()),
+                          "pfx_ndplan_execute");
+      backend_.sync();
+      return;
+    }
+    Complex *pc = reinterpret_cast<Complex *>(padded);
+    const std::size_t bytes = csize_ * sizeof(Complex);
+    backend_.execute_r2c_inplace(padded);
+    Complex *src, *dst;
+    if ((D - 1) % 2 == 0) {
+      src = pc;
+      dst = scratch_.data();
+    } else {
+      backend_.memcpy(scratch_.data(), pc, bytes);
+      src = scratch_.data();
+      dst = pc;
+    }
+    for (int stage = 1; stage < D; ++stage) {
+      backend_.memcpy(dst, src, bytes);  // one-rank exchange_local: identity
+      perform_fft_on_buffer(stage, D - 1 - stage, FFTDirection::Forward, dst);
+      std::swap(src, dst);
+    }
+    backend_.sync();
+  }
+
+  /// parafaft_r2c.hpp:473-542 (unnormalised unless scale != 1).
+  void backward_in_place(FloatType *padded, double scale = 1.0) {
+    if (mode_ == Mode::Fast) {
+      parafaft::check_pfx(pfx_ndplan_execute(nd_, PFX_BACKWARD, padded, scale, backend_.stream()),
+                          "pfx_ndplan_execute");
+      backend_.sync();
+      return;
+    }
+    Complex *pc = reinterpret_cast<Complex *>(padded);
+    const std::size_t bytes = csize_ * sizeof(Complex);
+    Complex *src, *dst;
+    if ((D - 1) % 2 == 0) {
+      src = pc;
+      dst = scratch_.data();
+    } else {
+      backend_.memcpy(scratch_.data(), pc, bytes);
+      src = scratch_.data();
+      dst = pc;
+    }
+    for (int stage = D - 1; stage >= 1; --stage) {
+      perform_fft_on_buffer(stage, D - 1 - stage, FFTDirection::Backward, src);
+      backend_.memcpy(dst, src, bytes);
+      std::swap(src, dst);
+    }
+    if (scale == 1.0) backend_.execute_c2r_inplace(padded);
+    else backend_.execute_c2r_inplace_scaled(padded, scale);
+    backend_.sync();
+  }
+
+  Backend &backend() { return backend_; }
+
+private:
+  std::size_t cprod(int from, int to) const {
+    std::size_t p = 1;
+    for (int i = from; i < to; ++i) p *= static_cast<std::size_t>(cplx_[i]);
+    return p;
+  }
+  std::size_t rprod(int from, int to) const {
+    std::size_t p = 1;
+    for (int i = from; i < to; ++i) p *= static_cast<std::size_t>(real_[i]);
+    return p;
+  }
+
+  void create_backend_plans() {
+    const std::size_t batch0 = rprod(0, D - 1);
+    const std::ptrdiff_t pd = 2 * static_cast<std::ptrdiff_t>(cplx_[D - 1]);
+    FloatType *plan_buf = reinterpret_cast<FloatType *>(scratch_.data());
+    backend_.create_r2c_inplace_plan(real_[D - 1], batch0, plan_buf, 1, pd);
+    backend_.create_c2r_inplace_plan(real_[D - 1], batch0, plan_buf, 1, pd);
+    for (int stage = 1; stage < D; ++stage) {
+      const int axis = D - 1 - stage;
+      const int length = cplx_[axis];
+      if (axis == 0) {
+        const std::size_t batch = cprod(1, D);
+        backend_.create_stage_plan(stage, length, batch, scratch_.data(),
+                                   static_cast<std::ptrdiff_t>(batch), 1);
+      } else {
+        const std::size_t trailing = cprod(axis + 1, D);
+        backend_.create_stage_plan(stage, length, trailing, scratch_.data(),
+                                   static_cast<std::ptrdiff_t>(trailing), 1);
+      }
+    }
+  }
+
+  void perform_fft_on_buffer(int stage, int axis, FFTDirection dir, Complex *buf) {
+    if (axis == 0) {
+      backend_.execute_stage(stage, dir, buf);
+    } else {
+      const std::size_t leading = cprod(0, axis);
+      const std::size_t slice = static_cast<std::size_t>(cplx_[axis]) * cprod(axis + 1, D);
+      for (std::size_t i = 0; i < leading; ++i) backend_.execute_stage(stage, dir, buf + i * slice);
+    }
+  }
+
+  Backend backend_;
+  Mode mode_;
+  int real_[D];
+  int cplx_[D];
+  std::size_t csize_ = 0;
+  ComplexBuffer scratch_;
+  pfx_ndplan nd_ = nullptr;
+};
+
+}  // namespace parafaft_mi355
+
+#endif  // PARAFAFT_MI355_ONE_RANK_HPP

@@ -1,0 +1,130 @@
+// ParaFaFT_C2C<D, MI355Backend<T>> and ParaFaFT_R2C<D, MI355Backend<T>> on one
+// rank over the reference's integration matrix (test/CMakeLists.txt:215-271):
+// shape in {Gaussian, step, LCG polynomial} x {D2 N32, D3 N32, D4 N16, D5 N8},
+// forward compared with the oracle at the reference tolerances (max-abs <
+// 1e-10 for double, test_fftw_c2c_nd.cpp:119; < 1e-4 * peak for float,
+// test_fftw_c2c_nd_float.cpp:101-107), then backward with 1/N^D scaling
+// against the input (test_fftw_c2c_roundtrip_nd.cpp:73-93).  Both the
+// faithful driver and the one-GPU fast path run.  Returns the failure count.
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <vector>
+
+#include "orc.h"
+#include "parafaft_mi355/parafaft_one_rank.hpp"
+
+using parafaft_mi355::Mode;
+
+static int failures = 0;
+static const char *kShape[] = {"gaussian", "step", "poly"};
+
+static void report(bool ok, const char *kind, const char *prec, int D, int N, int s, Mode m,
+                   double e1, double e2) {
+  std::printf("  %s %s D%d N%-3d %-8s %-8s fwd=%.2e roundtrip=%.2e %s\n", kind, prec, D, N,
+              kShape[s], m == Mode::Fast ? "fast" : "faithful", e1, e2, ok ? "[PASS]" : "[FAIL]");
+  if (!ok) ++failures;
+}
+
+static void sync_copy(void *dst, const void *src, size_t bytes) {
+  parafaft::check_pfx(pfx_memcpy_async(dst, src, bytes, nullptr), "copy");
+  parafaft::check_pfx(pfx_stream_synchronize(nullptr), "sync");
+}
+
+template <typename T, int D> static void c2c_case(int N, int shape_id, Mode mode) {
+  int shape[D];
+  size_t tot = 1;
+  for (int i = 0; i < D; ++i) {
+    shape[i] = N;
+    tot *= N;
+  }
+  std::vector<double> x(tot);
+  orc_generate_shape(x.data(), D, N, shape_id);
+  std::vector<std::complex<double>> ref(tot);
+  for (size_t i = 0; i < tot; ++i) ref[i] = x[i];
+  orc_parafaft_c2c(D, shape, reinterpret_cast<double *>(ref.data()), -1, 0);
+  double peak = 0;
+  for (auto &v : ref) peak = std::max(peak, std::abs(v));
+
+  using B = parafaft::MI355Backend<T>;
+  parafaft_mi355::ParaFaFT_C2C<D, B> fft(shape, parafaft::FFTPlanFlag::Estimate, mode);
+  typename B::ComplexBuffer d(fft.get_required_output_size());
+  std::vector<std::complex<T>> h(tot);
+  for (size_t i = 0; i < tot; ++i) h[i] = std::complex<T>(T(x[i]), 0);
+  sync_copy(d.data(), h.data(), tot * sizeof(h[0]));
+  fft.forward(d.data());
+  sync_copy(h.data(), d.data(), tot * sizeof(h[0]));
+  double e1 = 0;
+  for (size_t i = 0; i < tot; ++i) e1 = std::max(e1, std::abs(std::complex<double>(h[i]) - ref[i]));
+  fft.backward(d.data(), 1.0 / double(tot));
+  sync_copy(h.data(), d.data(), tot * sizeof(h[0]));
+  double e2 = 0;
+  for (size_t i = 0; i < tot; ++i) e2 = std::max(e2, std::abs(std::complex<double>(h[i]) - x[i]));
+  const bool dbl = sizeof(T) == 8;
+  const bool ok = dbl ? (e1 < 1e-10 && e2 < 1e-10) : (e1 < 1e-4 * peak && e2 < 1e-4);
+  report(ok, "C2C", dbl ? "double" : "float", D, N, shape_id, mode, e1, e2);
+}
+
+template <typename T, int D> static void r2c_case(int N, int shape_id, Mode mode) {
+  int shape[D];
+  size_t rows = 1;
+  for (int i = 0; i < D; ++i) shape[i] = N;
+  for (int i = 0; i < D - 1; ++i) rows *= N;
+  const int h = N / 2 + 1, pd = 2 * h;
+  std::vector<double> x(rows * N);
+  orc_generate_shape(x.data(), D, N, shape_id);
+  std::vector<double> ref(rows * pd, 0.0);
+  std::vector<T> pad(rows * pd, T(0));
+  for (size_t r = 0; r < rows; ++r)
+    for (int i = 0; i < N; ++i) {
+      pad[r * pd + i] = T(x[r * N + i]);
+      ref[r * pd + i] = x[r * N + i];
+    }
+  orc_parafaft_r2c(D, shape, ref.data(), -1, 0);
+  double peak = 0;
+  for (double v : ref) peak = std::max(peak, std::abs(v));
+
+  using B = parafaft::MI355Backend<T>;
+  parafaft_mi355::ParaFaFT_R2C<D, B> fft(shape, parafaft::FFTPlanFlag::Estimate, mode);
+  typename B::Buffer d(fft.get_required_output_size());
+  sync_copy(d.data(), pad.data(), pad.size() * sizeof(T));
+  fft.forward_in_place(d.data());
+  std::vector<T> out(pad.size());
+  sync_copy(out.data(), d.data(), out.size() * sizeof(T));
+  double e1 = 0;
+  for (size_t i = 0; i < out.size(); ++i) e1 = std::max(e1, std::abs(double(out[i]) - ref[i]));
+  fft.backward_in_place(d.data(), 1.0 / double(rows * N));
+  sync_copy(out.data(), d.data(), out.size() * sizeof(T));
+  double e2 = 0;
+  for (size_t r = 0; r < rows; ++r)
+    for (int i = 0; i < N; ++i) e2 = std::max(e2, std::abs(double(out[r * pd + i]) - x[r * N + i]));
+  const bool dbl = sizeof(T) == 8;
+  const bool ok = dbl ? (e1 < 1e-10 && e2 < 1e-10) : (e1 < 1e-4 * peak && e2 < 1e-4);
+  report(ok, "R2C", dbl ? "double" : "float", D, N, shape_id, mode, e1, e2);
+}
+
+template <typename T> static void matrix() {
+  for (Mode m : {Mode::Faithful, Mode::Fast})
+    for (int s = 0; s < 3; ++s) {
+      c2c_case<T, 2>(32, s, m);
+      c2c_case<T, 3>(32, s, m);
+      c2c_case<T, 4>(16, s, m);
+      c2c_case<T, 5>(8, s, m);
+      r2c_case<T, 2>(32, s, m);
+      r2c_case<T, 3>(32, s, m);
+      r2c_case<T, 4>(16, s, m);
+      r2c_case<T, 5>(8, s, m);
+    }
+}
+
+int main() {
+  try {
+    matrix<double>();
+    matrix<float>();
+  } catch (const std::exception &e) {
+    std::printf("exception: %s\n", e.what());
+    return 100;
+  }
+  std::printf("%d failure(s)\n", failures);
+  return failures;
+}
