@@ -32,7 +32,7 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, frac_log2: int = 4):
+def cpu_baseline(n: int, frac_log2: int = 3):
     """Time the FP64 oracle (restated FFTWBackend<double> batched plans) on a
     bounded sample of the workload: for each of the three axes, 1/2^frac_log2
     of that axis' n*n length-n transforms, laid out with that axis' stride
@@ -67,6 +67,9 @@ def main():
     ap.add_argument("--precision", default="f32")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a plain device copy of 4 GiB (the library's 2D-copy "
+                         "kernel) five times, to scale rocprof FETCH/WRITE_SIZE counters")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -79,11 +82,24 @@ def main():
     import parafaft_amd as P
     from parafaft_amd import _capi as A
 
+    L = A.load()
+    if args.calibrate_copy:
+        nbytes = 4 << 30
+        a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+        b = torch.empty_like(a)
+        a.fill_(1)
+        torch.cuda.synchronize()
+        for _ in range(5):
+            A.check(L.pfx_memcpy2d_async(b.data_ptr(), nbytes, a.data_ptr(), nbytes, nbytes, 1,
+                                         torch.cuda.current_stream().cuda_stream), "copy")
+        torch.cuda.synchronize()
+        print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 5}), flush=True)
+        return
+
     n = args.n
     D = 3
     N = n ** D
     prec = P.backend.precision_code(args.precision)
-    L = A.load()
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
