@@ -56,28 +56,29 @@ const Registry &registry() {
   return r;
 }
 
-// Environment override of the batch-contiguous tile: PFX_COL_TILE=<per_block>.
+// XCD-aware tile order for batch-contiguous kernels (PFX_XCD_REMAP=0 turns it
+// off, for measurement).
+// Both are read when a plan is created.
+bool xcd_remap_enabled() {
+  const char *s = getenv("PFX_XCD_REMAP");
+  return !(s && s[0] == '0');
+}
+
+// Override of the batch-contiguous tile: PFX_COL_TILE=<transforms per workgroup>.
 int col_tile_override() {
-  static int v = [] {
-    const char *s = getenv("PFX_COL_TILE");
-    return s ? atoi(s) : 0;
-  }();
-  return v;
+  const char *s = getenv("PFX_COL_TILE");
+  return s ? atoi(s) : 0;
 }
 
 const KernelEntry *find_kernel(int prec, int kind, int64_t n, bool col) {
   const KernelEntry *best = nullptr;
+  // Highest `preferred` wins unless PFX_COL_TILE names an available tile.
   const int want = col ? col_tile_override() : 0;
   for (const KernelEntry *e : registry().entries) {
     if (e->prec != prec || e->kind != kind || e->n != n) continue;
     if (kind == KIND_C2C && e->col != (col ? 1 : 0)) continue;
-    if (want > 0) {
-      if (e->per_block == want) return e;
-      if (!best && e->preferred) best = e;
-      continue;
-    }
-    if (e->preferred) return e;
-    if (!best) best = e;
+    if (want > 0 && e->per_block == want) return e;
+    if (!best || e->preferred > best->preferred) best = e;
   }
   return best;
 }
@@ -95,6 +96,7 @@ struct pfx_plan_t {
   int64_t inner = 0, outer = 1;
   int64_t stride = 1, dist = 0, dist_outer = 0;
   const KernelEntry *kernel = nullptr;  // null: direct-DFT fallback
+  int xcd_remap = 1;
   void *tw = nullptr;
   int64_t tw_len = 0;
 };
@@ -154,6 +156,7 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   p->stride = stride;
   p->dist = dist;
   p->dist_outer = dist_outer;
+  p->xcd_remap = xcd_remap_enabled() ? 1 : 0;
 
   const int bp = base_prec(precision);
   const size_t esz = bp ? 16 : 8;
@@ -251,6 +254,7 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
     a.scale = scale;
     a.use_scale = scale != 1.0;
     a.backward = direction == PFX_BACKWARD;
+    a.xcd_remap = p->xcd_remap;
     p->kernel->launch(a, s);
   } else {
     NaiveArgs a;

@@ -17,7 +17,7 @@ struct KernelEntry {
   int tpt;        // threads per transform
   int per_block;  // transforms per workgroup
   int lds_bytes;
-  int preferred;  // default choice among entries with the same key
+  int preferred;  // priority among entries with the same key (highest is the default)
   const char *radices;
   LaunchFn launch;
 };
@@ -28,6 +28,11 @@ template <typename T, int TPT, int C, bool COL, int... Rs>
 void launch_c2c(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
   c2c_kernel<T, TPT, C, COL, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+}
+template <typename T, int TPT, int C, int... Rs>
+void launch_c2c_wide(const StockhamArgs &a, hipStream_t s) {
+  const int64_t nb = (a.total + C - 1) / C;
+  c2c_wide_kernel<T, TPT, C, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, int... Rs>
 void launch_r2c(const StockhamArgs &a, hipStream_t s) {

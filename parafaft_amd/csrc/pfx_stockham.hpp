@@ -33,6 +33,8 @@ namespace pfx {
 
 enum Kind { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
 
+template <int R, int... Rest> constexpr int first_of() { return R; }
+
 struct StockhamArgs {
   void* data;            // complex view of the buffer (R2C/C2R: padded rows)
   const void* tw;        // W_L^i, i in [0, L): L = N (C2C) or 2N (R2C/C2R)
@@ -44,7 +46,19 @@ struct StockhamArgs {
   double scale;          // multiplies every output (1.0 = none)
   int backward;          // C2C only: 1 = exp(+i...), via re/im swap
   int use_scale;
+  int xcd_remap;         // 1: give each XCD a contiguous run of tiles
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
+// XCD i % 8), and each XCD has its own L2.  Batch-contiguous tiles that share
+// a cache line must run on the same XCD, so tile ids are remapped to give XCD
+// x the contiguous range [x q + min(x, r), ...) of the nb = 8 q + r tiles.
+constexpr int kNumXcd = 8;
+__device__ inline int64_t xcd_tile(int64_t i, int64_t nb) {
+  const int64_t q = nb / kNumXcd, r = nb % kNumXcd;
+  const int64_t x = i % kNumXcd, k = i / kNumXcd;
+  return x * q + (x < r ? x : r) + k;
+}
 
 template <typename T, int N, int C, bool COL> struct LdsLayout {
   // ROW: one padded row of N per transform; one pad slot every 16 entries
@@ -190,7 +204,9 @@ c2c_kernel(StockhamArgs a) {
   __shared__ V2<T> lds[L::size];
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
-  const int64_t b = (int64_t)blockIdx.x * C + c;
+  int64_t tile = blockIdx.x;
+  if (COL && a.xcd_remap) tile = xcd_tile(tile, gridDim.x);
+  const int64_t b = tile * C + c;
   GlobalPtr<T> g;
   g.valid = b < a.total;
   g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
@@ -299,6 +315,127 @@ c2r_kernel(StockhamArgs a) {
   }
   __syncthreads();
   Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
+}
+
+}  // namespace pfx
+
+namespace pfx {
+
+// ---------------------------------------------------------------------------
+// Wide batch-contiguous C2C: C transforms per workgroup (C * 8 bytes of every
+// row in one run, 256 B for C = 32 complex<float>), each thread holding
+// E = N / TPT elements in registers across all passes.  The LDS exchange
+// between passes moves one component at a time (re, then im), so the tile
+// needs N * C scalars of LDS rather than N * C complex values.
+// Requires every radix to divide E (all passes are exact).
+// ---------------------------------------------------------------------------
+template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns, int... Rs>
+struct WidePasses;
+
+template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns>
+struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns> {
+  __device__ static inline void run(int, int, V2<T>*, T*, const V2<T>*, const GlobalPtr<T>&) {}
+};
+
+template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns, int R, int... Rest>
+struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
+  static_assert(E % R == 0, "radix must divide the per-thread element count");
+  __device__ static inline void run(int j, int c, V2<T>* v, T* lds, const V2<T>* tw,
+                                    const GlobalPtr<T>& g) {
+    constexpr int Q = E / R;
+    // twiddle + butterflies on the values this thread holds
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+      const int jj = j + q * TPT;
+      if constexpr (Ns > 1) {
+        const int kk = jj % Ns;
+        constexpr int TSTEP = (N / (Ns * R)) * TWS;
+#pragma unroll
+        for (int m = 1; m < R; ++m) v[q * R + m] = cmul<T>(v[q * R + m], tw[kk * m * TSTEP]);
+      }
+      bfly<T, R>(v + q * R);
+    }
+    if constexpr (sizeof...(Rest) == 0) {
+      // last pass: outputs straight to HBM
+#pragma unroll
+      for (int q = 0; q < Q; ++q) {
+        const int jj = j + q * TPT;
+        const int ob = (jj / Ns) * Ns * R + jj % Ns;
+#pragma unroll
+        for (int m = 0; m < R; ++m) {
+          V2<T> x = v[q * R + m];
+          if (g.use_scale) x = scl<T>(x, g.scale);
+          if (g.swap_io) x = swp<T>(x);
+          if (g.valid) g.p[(int64_t)(ob + m * Ns) * g.stride] = x;
+        }
+      }
+    } else {
+      constexpr int R2 = first_of<Rest...>();
+      constexpr int Q2 = E / R2;
+      constexpr int NB2 = N / R2;
+      V2<T> w[E];
+#pragma unroll
+      for (int comp = 0; comp < 2; ++comp) {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const int jj = j + q * TPT;
+          const int ob = (jj / Ns) * Ns * R + jj % Ns;
+#pragma unroll
+          for (int m = 0; m < R; ++m)
+            lds[(ob + m * Ns) * C + c] = comp ? v[q * R + m].y : v[q * R + m].x;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q2; ++q) {
+          const int jj = j + q * TPT;
+#pragma unroll
+          for (int m = 0; m < R2; ++m) {
+            const T s = lds[(jj + m * NB2) * C + c];
+            if (comp) w[q * R2 + m].y = s;
+            else w[q * R2 + m].x = s;
+          }
+        }
+        __syncthreads();
+      }
+#pragma unroll
+      for (int i = 0; i < E; ++i) v[i] = w[i];
+      WidePasses<T, N, TPT, C, E, TWS, PI + 1, Ns * R, Rest...>::run(j, c, v, lds, tw, g);
+    }
+  }
+};
+
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C)
+c2c_wide_kernel(StockhamArgs a) {
+  constexpr int N = (Rs * ... * 1);
+  constexpr int E = N / TPT;
+  constexpr int R0 = first_of<Rs...>();
+  constexpr int Q0 = E / R0;
+  constexpr int NB0 = N / R0;
+  __shared__ T lds[N * C];
+  const int c = threadIdx.x % C, j = threadIdx.x / C;
+  int64_t tile = blockIdx.x;
+  if (a.xcd_remap) tile = xcd_tile(tile, gridDim.x);
+  const int64_t b = tile * C + c;
+  GlobalPtr<T> g;
+  g.valid = b < a.total;
+  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  g.stride = a.stride;
+  g.swap_io = a.backward != 0;
+  g.scale = T(a.scale);
+  g.use_scale = a.use_scale != 0;
+  V2<T> v[E];
+#pragma unroll
+  for (int q = 0; q < Q0; ++q) {
+    const int jj = j + q * TPT;
+#pragma unroll
+    for (int m = 0; m < R0; ++m) {
+      V2<T> x = mk<T>(T(0), T(0));
+      if (g.valid) x = g.p[(int64_t)(jj + m * NB0) * g.stride];
+      v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
+    }
+  }
+  WidePasses<T, N, TPT, C, E, 1, 0, 1, Rs...>::run(j, c, v, lds, static_cast<const V2<T>*>(a.tw), g);
 }
 
 }  // namespace pfx

@@ -174,3 +174,33 @@ def test_scaled_epilogue(prec):
     plan.execute(1, d, scale=1.0 / n)
     y = to_host(d)
     assert rel_l2(y, x) <= tol(prec, n)
+
+
+@pytest.mark.parametrize("prec,tile", [("f32", 8), ("f32", 16), ("f32", 32), ("f32", 64),
+                                       ("f64", 4), ("f64", 8), ("f64", 16)])
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048])
+def test_column_tile_variants(prec, tile, n, monkeypatch):
+    """Every batch-contiguous tile variant (plain LDS exchange, large tile,
+    wide per-component exchange) against the oracle, with a ragged batch."""
+    import parafaft_amd as P
+    monkeypatch.setenv("PFX_COL_TILE", str(tile))
+    batch = 3 * tile + 5
+    x = O.uniform_complex((n * batch,), seed=n + tile).astype(_cdtype(prec))
+    plan = P.Plan.c2c(prec, n, batch, batch, 1)
+    name, per_block, _ = plan.describe()
+    for direction, sign in ((0, -1), (1, +1)):
+        d = to_dev(x)
+        plan.execute(direction, d)
+        y = to_host(d)
+        ref = _ref_c2c(x, n, batch, batch, 1, sign)
+        assert rel_l2(y, ref) <= tol(prec, n), (name, per_block, direction, rel_l2(y, ref))
+    # middle-axis form with the same variant
+    lead = 3
+    xm = O.uniform_complex((lead * n * batch,), seed=7 + n).astype(_cdtype(prec))
+    ym = _run_c2c(prec, n, batch, batch, 1, 0, xm, outer=lead, outer_dist=n * batch)
+    ref = xm.astype(np.complex128).copy().reshape(lead, n * batch)
+    for i in range(lead):
+        row = np.ascontiguousarray(ref[i])
+        O.c2c_batched(row, n, batch, batch, 1, -1)
+        ref[i] = row
+    assert rel_l2(ym, ref) <= tol(prec, n), name
