@@ -107,8 +107,10 @@ def main():
     log(f"[bench] allocating {N * 8 / 2**30:.1f} GiB complex<float> {n}^3")
     g = torch.Generator(device=dev)
     g.manual_seed(20241203)
-    x = (torch.rand((N, 2), device=dev, generator=g, dtype=torch.float32) * 2 - 1)
+    rdt = torch.float64 if prec == A.PFX_F64 else torch.float32
+    x = (torch.rand((N, 2), device=dev, generator=g, dtype=rdt) * 2 - 1)
     x = torch.view_as_complex(x).reshape(n, n, n).contiguous()
+    esz = x.element_size()
 
     nd = C.c_void_p()
     shape = (C.c_int * 3)(n, n, n)
@@ -127,7 +129,7 @@ def main():
 
     # ---- accuracy (outside the timed region): FP32 vs FP64 on the same input
     acc = {}
-    if not args.no_check:
+    if not args.no_check and prec != A.PFX_F64:
         log("[bench] accuracy check vs FP64 path")
         x64 = x.to(torch.complex128)
         nd64 = C.c_void_p()
@@ -178,10 +180,10 @@ def main():
 
     flops = 5.0 * N * math.log2(N)
     gflops = flops / (ms_per_step * 1e-3) / 1e9
-    alg_bytes = 2.0 * D * N * 8
+    alg_bytes = 2.0 * D * N * esz
     tbs = alg_bytes / (ms_per_step * 1e-3) / 1e12
     dom = max(range(D), key=lambda s: stage_ms[s])
-    per_launch = 2.0 * N * 8
+    per_launch = 2.0 * N * esz
     achieved = per_launch / (stage_ms[dom] * 1e-3) / 1e9
 
     traffic = None
@@ -214,7 +216,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if prec in (A.PFX_F32,) else args.precision,
+        "dtype": {A.PFX_F32: "fp32", A.PFX_F64: "fp64"}.get(prec, args.precision),
         "data": "synthetic: iid Uniform[-1,1) re/im (seeded), resident in HBM",
         "config": {"workload": f"3D C2C forward, in place, {n}^3 complex<float>, one GPU "
                                "(one-rank fast path: one launch per axis)",

@@ -40,6 +40,10 @@ constexpr double C5_1 = 0.30901699437494742410229341718281906;   // cos(2pi/5)
 constexpr double C5_2 = -0.80901699437494742410229341718281906;  // cos(4pi/5)
 constexpr double S5_1 = 0.95105651629515357211643933337938214;   // sin(2pi/5)
 constexpr double S5_2 = 0.58778525229247312916870595463907277;   // sin(4pi/5)
+constexpr double COS_PI16 = 0.98078528040323044912618223613424369;
+constexpr double SIN_PI16 = 0.19509032201612826784828486847702224;
+constexpr double COS_3PI16 = 0.83146961230254523707878837761790576;
+constexpr double SIN_3PI16 = 0.55557023301960222474283081394853287;
 }  // namespace k
 
 template <typename T> __host__ __device__ inline void bfly2(V2<T>* v) {
@@ -136,6 +140,41 @@ template <typename T> __host__ __device__ inline void bfly16(V2<T>* v) {
   for (int i = 0; i < 16; ++i) v[i] = t[i];
 }
 
+// radix-32 = 2 x radix-16 (decimation in time): X[k] = E[k] + W32^k O[k],
+// X[k + 16] = E[k] - W32^k O[k], W32^k = cos(pi k/16) - i sin(pi k/16).
+template <typename T> __host__ __device__ inline void bfly32(V2<T>* v) {
+  const T c1 = T(k::COS_PI16), s1 = T(k::SIN_PI16);
+  const T c2 = T(k::COS_PI8), s2 = T(k::SIN_PI8);
+  const T c3 = T(k::COS_3PI16), s3 = T(k::SIN_3PI16);
+  const T r = T(k::SQRT1_2);
+  V2<T> e[16], o[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    e[i] = v[2 * i];
+    o[i] = v[2 * i + 1];
+  }
+  bfly16<T>(e);
+  bfly16<T>(o);
+  // (cos, sin) of pi k / 16 for k = 1..15; the twiddle is cos - i sin
+  const T cs[16][2] = {{T(1), T(0)}, {c1, s1}, {c2, s2}, {c3, s3}, {r, r},    {s3, c3},
+                       {s2, c2},     {s1, c1}, {T(0), T(1)}, {-s1, c1}, {-s2, c2}, {-s3, c3},
+                       {-r, r},      {-c3, s3}, {-c2, s2}, {-c1, s1}};
+#pragma unroll
+  for (int q = 1; q < 16; ++q) {
+    if (q == 8) {
+      o[q] = mul_mi<T>(o[q]);
+    } else {
+      const T c = cs[q][0], s = cs[q][1];
+      o[q] = mk<T>(o[q].x * c + o[q].y * s, o[q].y * c - o[q].x * s);
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    v[q] = add<T>(e[q], o[q]);
+    v[q + 16] = sub<T>(e[q], o[q]);
+  }
+}
+
 template <typename T, int R> __host__ __device__ inline void bfly(V2<T>* v) {
   if constexpr (R == 2) bfly2<T>(v);
   else if constexpr (R == 3) bfly3<T>(v);
@@ -143,6 +182,7 @@ template <typename T, int R> __host__ __device__ inline void bfly(V2<T>* v) {
   else if constexpr (R == 5) bfly5<T>(v);
   else if constexpr (R == 8) bfly8<T>(v);
   else if constexpr (R == 16) bfly16<T>(v);
+  else if constexpr (R == 32) bfly32<T>(v);
   else static_assert(R == 2, "unsupported radix");
 }
 

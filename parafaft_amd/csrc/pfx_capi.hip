@@ -70,6 +70,10 @@ int col_tile_override() {
   return s ? atoi(s) : 0;
 }
 
+// Kernel-choice override for measurement: PFX_KERNEL_MATCH=<substring of the
+// radix description>, e.g. "wide:32, 32".
+const char *kernel_match_override() { return getenv("PFX_KERNEL_MATCH"); }
+
 const KernelEntry *find_kernel(int prec, int kind, int64_t n, bool col) {
   const KernelEntry *best = nullptr;
   // Highest `preferred` wins unless PFX_COL_TILE names an available tile.
@@ -77,6 +81,8 @@ const KernelEntry *find_kernel(int prec, int kind, int64_t n, bool col) {
   for (const KernelEntry *e : registry().entries) {
     if (e->prec != prec || e->kind != kind || e->n != n) continue;
     if (kind == KIND_C2C && e->col != (col ? 1 : 0)) continue;
+    const char *m = kernel_match_override();
+    if (m && *m && strstr(e->radices, m)) return e;
     if (want > 0 && e->per_block == want) return e;
     if (!best || e->preferred > best->preferred) best = e;
   }
@@ -98,6 +104,7 @@ struct pfx_plan_t {
   const KernelEntry *kernel = nullptr;  // null: direct-DFT fallback
   int xcd_remap = 1;
   void *tw = nullptr;
+  void *ptw = nullptr;  // per-pass table of the tiled kernel
   int64_t tw_len = 0;
 };
 
@@ -196,9 +203,14 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
     return hip_fail(e, "hipMalloc twiddles");
   }
   e = make_twiddles(bp, p->tw_len, p->tw, nullptr);
+  if (e == hipSuccess && p->kernel) {
+    e = hipMalloc(&p->ptw, (size_t)p->kernel->n * esz);
+    if (e == hipSuccess) e = make_pass_twiddles(bp, p->kernel->rad, p->ptw, nullptr);
+  }
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) {
     hipFree(p->tw);
+    if (p->ptw) hipFree(p->ptw);
     delete p;
     return hip_fail(e, "twiddle generation");
   }
@@ -245,6 +257,7 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
     StockhamArgs a;
     a.data = data;
     a.tw = p->tw;
+    a.ptw = p->ptw;
     a.total = total;
     a.inner = p->inner;
     a.stride = p->stride;
@@ -284,6 +297,7 @@ int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream) {
 int pfx_plan_destroy(pfx_plan p) {
   if (!p) return PFX_SUCCESS;
   if (p->tw) hipFree(p->tw);
+  if (p->ptw) hipFree(p->ptw);
   delete p;
   return PFX_SUCCESS;
 }

@@ -9,6 +9,12 @@ namespace pfx {
 
 using LaunchFn = void (*)(const StockhamArgs &, hipStream_t);
 
+struct Radices {
+  int n;
+  int r[8];
+};
+template <int... Rs> constexpr Radices radices() { return Radices{(int)sizeof...(Rs), {Rs...}}; }
+
 struct KernelEntry {
   int prec;       // PFX_F32 / PFX_F64
   int kind;       // KIND_C2C / KIND_R2C / KIND_C2R
@@ -20,6 +26,7 @@ struct KernelEntry {
   int preferred;  // priority among entries with the same key (highest is the default)
   const char *radices;
   LaunchFn launch;
+  Radices rad;    // pass radices, first pass first
 };
 
 template <int... Rs> constexpr int prod() { return (Rs * ... * 1); }
@@ -74,6 +81,9 @@ int naive_max_length(int prec);
 // Twiddle table generation on the device: W_L^i = exp(-2 pi i i / L), i < L,
 // computed with sincospi in double and rounded to the table precision.
 hipError_t make_twiddles(int prec, int64_t L, void *dst, hipStream_t s);
+// Per-pass Stockham table for an FFT of length N = prod(rad): entry
+// m * Ns + kk - 1 = W_{Ns R}^{kk m} for the pass (Ns, R), N - 1 entries.
+hipError_t make_pass_twiddles(int prec, const Radices &rad, void *dst, hipStream_t s);
 
 // Copy kernels.
 struct PackSeg {

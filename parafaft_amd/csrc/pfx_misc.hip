@@ -34,6 +34,63 @@ __global__ void twiddle_kernel(V2<T>* out, int64_t L) {
   out[i] = mk<T>(T(c), T(-s));
 }
 
+// exp(-2 pi i e / L) for 0 <= e < L with quarter-turn reduction.
+__device__ inline void root_of_unity(int64_t e, int64_t L, double& c, double& s) {
+  if ((L & 3) == 0) {
+    const int64_t q = L / 4;
+    const int64_t quad = e / q, r = e - quad * q;
+    double sr, cr;
+    sincospi(2.0 * (double)r / (double)L, &sr, &cr);
+    switch (quad & 3) {
+      case 0: s = sr; c = cr; break;
+      case 1: s = cr; c = -sr; break;
+      case 2: s = -sr; c = -cr; break;
+      default: s = -cr; c = sr; break;
+    }
+  } else {
+    sincospi(2.0 * (double)e / (double)L, &s, &c);
+  }
+}
+
+struct PassTwArgs {
+  int n;        // number of passes
+  int ns[8];    // span before pass p
+  int r[8];     // radix of pass p
+  int N;
+};
+
+template <typename T>
+__global__ void pass_twiddle_kernel(V2<T>* out, PassTwArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.N - 1) return;
+  const int t = i + 1;  // = m * Ns + kk for the last pass with Ns <= t
+  int p = 0;
+  for (int q = 0; q < a.n; ++q)
+    if (a.ns[q] <= t) p = q;
+  const int Ns = a.ns[p], L = Ns * a.r[p];
+  const int m = t / Ns, kk = t - m * Ns;
+  double c, s;
+  root_of_unity((int64_t)kk * m % L, L, c, s);
+  out[i] = mk<T>(T(c), T(-s));
+}
+
+hipError_t make_pass_twiddles(int prec, const Radices& rad, void* dst, hipStream_t s) {
+  PassTwArgs a;
+  a.n = rad.n;
+  int ns = 1;
+  for (int p = 0; p < rad.n; ++p) {
+    a.ns[p] = ns;
+    a.r[p] = rad.r[p];
+    ns *= rad.r[p];
+  }
+  a.N = ns;
+  if (a.N < 2) return hipSuccess;
+  const unsigned nb = (unsigned)((a.N - 1 + 255) / 256);
+  if (prec == 1) pass_twiddle_kernel<double><<<nb, 256, 0, s>>>(static_cast<double2*>(dst), a);
+  else pass_twiddle_kernel<float><<<nb, 256, 0, s>>>(static_cast<float2*>(dst), a);
+  return hipGetLastError();
+}
+
 hipError_t make_twiddles(int prec, int64_t L, void* dst, hipStream_t s) {
   const int bs = 256;
   const unsigned nb = (unsigned)((L + bs - 1) / bs);

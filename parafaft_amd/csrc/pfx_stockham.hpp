@@ -38,6 +38,8 @@ template <int R, int... Rest> constexpr int first_of() { return R; }
 struct StockhamArgs {
   void* data;            // complex view of the buffer (R2C/C2R: padded rows)
   const void* tw;        // W_L^i, i in [0, L): L = N (C2C) or 2N (R2C/C2R)
+  const void* ptw;       // per-pass table of the LDS FFT: entry m*Ns + kk - 1
+                         // holds W_{Ns R}^{kk m} for the pass with span Ns
   int64_t total;         // number of transforms = inner * outer
   int64_t inner;         // transforms per outer slice
   int64_t stride;        // element stride inside a transform (complex units)
@@ -73,8 +75,8 @@ template <typename T, int N, int C, bool COL> struct LdsLayout {
 
 // One Stockham pass executed by one thread for its Q butterflies.
 // SRC/DST: 0 = LDS, 1 = HBM.
-template <typename T, int N, int R, int Ns, int TPT, int TWS, class LoadF, class StoreF>
-__device__ inline void stockham_pass(int j, const V2<T>* __restrict__ tw, LoadF load, StoreF store) {
+template <typename T, int N, int R, int Ns, int TPT, class LoadF, class StoreF>
+__device__ inline void stockham_pass(int j, const V2<T>* __restrict__ stw, LoadF load, StoreF store) {
   constexpr int NB = N / R;                  // butterflies in this pass
   constexpr int Q = (NB + TPT - 1) / TPT;    // butterflies per thread
   constexpr bool EXACT = (NB % TPT) == 0;
@@ -93,9 +95,8 @@ __device__ inline void stockham_pass(int j, const V2<T>* __restrict__ tw, LoadF 
     if (EXACT || jj < NB) {
       if constexpr (Ns > 1) {
         const int kk = jj % Ns;
-        constexpr int TSTEP = (N / (Ns * R)) * TWS;
 #pragma unroll
-        for (int m = 1; m < R; ++m) v[q][m] = cmul<T>(v[q][m], tw[kk * m * TSTEP]);
+        for (int m = 1; m < R; ++m) v[q][m] = cmul<T>(v[q][m], stw[m * Ns + kk - 1]);
       }
       bfly<T, R>(v[q]);
     }
@@ -172,13 +173,25 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
       }
     };
     if constexpr (IS_LAST && LAST_G) {
-      stockham_pass<T, N, R, Ns, TPT, TWS>(j, tw, load, StoreG{&g});
+      stockham_pass<T, N, R, Ns, TPT>(j, tw, load, StoreG{&g});
     } else {
-      stockham_pass<T, N, R, Ns, TPT, TWS>(j, tw, load, StoreLds{lds, c});
+      stockham_pass<T, N, R, Ns, TPT>(j, tw, load, StoreLds{lds, c});
     }
     Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI + 1, Ns * R, Rest...>::run(j, c, lds, tw, g);
   }
 };
+
+// Copy the per-pass twiddle table (N - 1 entries) into LDS; only plans with
+// more than one pass multiply by twiddles.
+template <typename T, int N, int NPASS>
+__device__ inline const V2<T>* stage_twiddles(V2<T>* stw, const void* ptw) {
+  if constexpr (NPASS > 1) {
+    const V2<T>* src = static_cast<const V2<T>*>(ptw);
+    for (int i = threadIdx.x; i < N - 1; i += blockDim.x) stw[i] = src[i];
+    __syncthreads();
+  }
+  return stw;
+}
 
 template <int TPT, int C, bool COL>
 __device__ inline void thread_coords(int& j, int& c) {
@@ -202,6 +215,8 @@ c2c_kernel(StockhamArgs a) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, COL>;
   __shared__ V2<T> lds[L::size];
+  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
   int64_t tile = blockIdx.x;
@@ -214,7 +229,6 @@ c2c_kernel(StockhamArgs a) {
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
-  const V2<T>* tw = static_cast<const V2<T>*>(a.tw);
   Passes<T, N, TPT, C, COL, 1, true, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
 }
 
@@ -230,6 +244,8 @@ r2c_kernel(StockhamArgs a) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
   __shared__ V2<T> lds[L::size];
+  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = (int64_t)blockIdx.x * C + c;
@@ -241,7 +257,7 @@ r2c_kernel(StockhamArgs a) {
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
   const V2<T>* tw = static_cast<const V2<T>*>(a.tw);  // W_{2N}^i, i < 2N
-  Passes<T, N, TPT, C, false, 2, true, false, 0, 1, Rs...>::run(j, c, lds, tw, g);
+  Passes<T, N, TPT, C, false, 2, true, false, 0, 1, Rs...>::run(j, c, lds, ptw, g);
   // post-twiddle: pairs (k, N-k) for k = 0..N/2
   const T half = T(0.5);
   for (int k = j; k <= N / 2; k += TPT) {
@@ -281,6 +297,8 @@ c2r_kernel(StockhamArgs a) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
   __shared__ V2<T> lds[L::size];
+  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = (int64_t)blockIdx.x * C + c;
@@ -314,7 +332,7 @@ c2r_kernel(StockhamArgs a) {
     }
   }
   __syncthreads();
-  Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
+  Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, ptw, g);
 }
 
 }  // namespace pfx
@@ -349,9 +367,8 @@ struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
       const int jj = j + q * TPT;
       if constexpr (Ns > 1) {
         const int kk = jj % Ns;
-        constexpr int TSTEP = (N / (Ns * R)) * TWS;
 #pragma unroll
-        for (int m = 1; m < R; ++m) v[q * R + m] = cmul<T>(v[q * R + m], tw[kk * m * TSTEP]);
+        for (int m = 1; m < R; ++m) v[q * R + m] = cmul<T>(v[q * R + m], tw[m * Ns + kk - 1]);
       }
       bfly<T, R>(v + q * R);
     }
@@ -413,6 +430,8 @@ c2c_wide_kernel(StockhamArgs a) {
   constexpr int Q0 = E / R0;
   constexpr int NB0 = N / R0;
   __shared__ T lds[N * C];
+  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   const int c = threadIdx.x % C, j = threadIdx.x / C;
   int64_t tile = blockIdx.x;
   if (a.xcd_remap) tile = xcd_tile(tile, gridDim.x);
@@ -435,7 +454,7 @@ c2c_wide_kernel(StockhamArgs a) {
       v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
     }
   }
-  WidePasses<T, N, TPT, C, E, 1, 0, 1, Rs...>::run(j, c, v, lds, static_cast<const V2<T>*>(a.tw), g);
+  WidePasses<T, N, TPT, C, E, 1, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
 }
 
 }  // namespace pfx

@@ -204,3 +204,24 @@ def test_column_tile_variants(prec, tile, n, monkeypatch):
         O.c2c_batched(row, n, batch, batch, 1, -1)
         ref[i] = row
     assert rel_l2(ym, ref) <= tol(prec, n), name
+
+
+@pytest.mark.parametrize("match,prec,n", [("wide:32, 32", "f32", 1024),
+                                          ("wide:16, 16, 4", "f32", 1024),
+                                          ("wide:16, 16, 8", "f32", 2048),
+                                          ("wide:16, 16, 4", "f64", 1024),
+                                          ("wide:8, 8, 8", "f64", 512)])
+def test_named_column_kernels(match, prec, n, monkeypatch):
+    """Kernels selected by radix description (the measurement override)."""
+    import parafaft_amd as P
+    monkeypatch.setenv("PFX_KERNEL_MATCH", match)
+    batch = 70
+    x = O.uniform_complex((n * batch,), seed=n + len(match)).astype(_cdtype(prec))
+    plan = P.Plan.c2c(prec, n, batch, batch, 1)
+    name = plan.describe()[0]
+    assert match in name, name
+    for direction, sign in ((0, -1), (1, +1)):
+        d = to_dev(x)
+        plan.execute(direction, d)
+        ref = _ref_c2c(x, n, batch, batch, 1, sign)
+        assert rel_l2(to_host(d), ref) <= tol(prec, n), (name, direction)
