@@ -58,6 +58,70 @@ def cpu_baseline(n: int, frac_log2: int = 3):
                        f"strides 1, {n} and {m} elements; {dt:.2f} s")}
 
 
+def bench_pencil(args, torch, A):
+    """Times ParaFaFT_R2C<3>::forward_in_place for rank (0,0) of a 2048^3 real
+    grid on a 4x4 process grid: R2C on the 512x512x2048 share, then per
+    transition pack -> self copy -> unpack -> C2C, peers' blocks resident."""
+    from parafaft_amd.pencil import PencilShareR2C
+    share = PencilShareR2C([2048] * 3, [4, 4], [0, 0], precision=args.precision)
+    rdt = torch.float64 if share.esz == 16 else torch.float32
+    buf = torch.zeros(share.get_required_output_size(), dtype=rdt, device="cuda")
+    g = torch.Generator(device="cuda")
+    g.manual_seed(3)
+    real = torch.randn((512 * 512, 2048), generator=g, device="cuda", dtype=rdt)
+    buf[:512 * 512 * 2050].view(512 * 512, 2050)[:, :2048] = real
+    del real
+    # peers' incoming blocks: seeded synthetic values, resident on the device
+    for t, geo in enumerate(share.geoms):
+        for p in range(len(geo.recv_counts)):
+            if p != share.me[t]:
+                addr, cnt = share.recv_segment(t, p)
+                blk = torch.randn(2 * cnt, generator=g, device="cuda", dtype=rdt)
+                A.check(A.load().pfx_memcpy_async(addr, blk.data_ptr(), blk.numel() *
+                                                  blk.element_size(), None), "peer fill")
+                torch.cuda.synchronize()
+                del blk
+    torch.cuda.synchronize()
+    labels = ["r2c", "pack0", "unpack0", "fft1", "pack1", "unpack1", "fft2", "end"]
+    stream = torch.cuda.current_stream()
+    share.stream = stream.cuda_stream
+    for _ in range(args.warmup):
+        share.forward_in_place(buf)
+    torch.cuda.synchronize()
+    evs = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        marks = {}
+
+        def mark(label, marks=marks):
+            e = torch.cuda.Event(enable_timing=True)
+            e.record(stream)
+            marks[label] = e
+        share.forward_in_place(buf, events=mark)
+        evs.append(marks)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    parts = {}
+    for a, b in zip(labels[:-1], labels[1:]):
+        parts[a] = sum(m[a].elapsed_time(m[b]) for m in evs) / args.steps
+    esz = share.esz
+    real_bytes = 512 * 512 * 2048 * (esz // 2)
+    cplx = 512 * 512 * 1025 * esz
+    cplx2 = 512 * 2048 * 257 * esz
+    alg = {"r2c": real_bytes + cplx, "pack0": 2 * cplx, "unpack0": 2 * cplx2, "fft1": 2 * cplx2,
+           "pack1": 2 * cplx2, "unpack1": 2 * cplx2, "fft2": 2 * cplx2}
+    total = sum(alg.values())
+    print(json.dumps({
+        "workload": "R2C pencil share: rank (0,0) of 2048^3 on a 4x4 grid, 512x512x2048 real, "
+                    "pack/unpack into 4+4 device segments per transition",
+        "precision": args.precision, "steps": args.steps, "ms_per_forward": round(ms, 4),
+        "part_ms": {k: round(v, 4) for k, v in parts.items()},
+        "part_tbs": {k: round(alg[k] / (parts[k] * 1e-3) / 1e12, 3) for k in parts},
+        "algorithmic_bytes": total, "tbs_algorithmic": round(total / (ms * 1e-3) / 1e12, 3),
+        "gflops_3d": round(2.5 * 2048 ** 3 / 16 * math.log2(2048 ** 3) / (ms * 1e-3) / 1e9, 1)}),
+        flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -67,6 +131,9 @@ def main():
     ap.add_argument("--precision", default="f32")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--workload", default="cube", choices=["cube", "pencil"],
+                    help="cube: the 1024^3 C2C headline; pencil: one rank's share of a "
+                         "2048^3 R2C on a 4x4 grid with on-device pack/unpack")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 4 GiB (the library's 2D-copy "
                          "kernel) five times, to scale rocprof FETCH/WRITE_SIZE counters")
@@ -95,6 +162,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 5}), flush=True)
         return
+
+    if args.workload == "pencil":
+        return bench_pencil(args, torch, A)
 
     n = args.n
     D = 3

@@ -1,0 +1,187 @@
+"""One rank's ParaFaFT_R2C pencil share on one GPU (parafaft_r2c.hpp:360-430 with
+exchange_packed's pack/unpack, parafaft_common.hpp:564-669), peers' blocks
+left on the device as seeded synthetic segments.
+
+- pack and unpack are checked bit-exact against the oracle's restatement of
+  the reference's loops applied to the GPU's own intermediate data;
+- the whole share transform is checked against the oracle pipeline
+  (FP64 restated R2C / pack / unpack / C2C) on scaled-down grids, and against
+  the FP64 GPU path at the full 2048^3-on-4x4 share (512 x 512 x 2048).
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import rel_l2, to_dev, to_host, torch
+
+pytestmark = pytest.mark.gpu
+
+
+def filtered_noise(shape, seed):
+    """iid N(0,1) white noise filtered to P(k) ~ k^-3 on the share treated as
+    a periodic box (numpy, FP64; data generation only)."""
+    rng = np.random.default_rng(seed)
+    x = rng.standard_normal(shape)
+    X = np.fft.rfftn(x)
+    ks = np.meshgrid(*[np.fft.fftfreq(n) * n for n in shape[:-1]],
+                     np.fft.rfftfreq(shape[-1]) * shape[-1], indexing="ij")
+    k = np.sqrt(sum(kk ** 2 for kk in ks))
+    amp = np.zeros_like(k)
+    amp[k > 0] = k[k > 0] ** -1.5
+    return np.fft.irfftn(X * amp, s=shape, axes=list(range(len(shape))))
+
+
+def _fill_peers(share, seed, cdt):
+    """Seeded synthetic blocks in every peer receive segment; returns them."""
+    peers = {}
+    for t, g in enumerate(share.geoms):
+        for p in range(len(g.recv_counts)):
+            if p == share.me[t]:
+                continue
+            vals = O.uniform_complex((g.recv_counts[p],), seed=seed + 100 * t + p).astype(cdt)
+            addr, count = share.recv_segment(t, p)
+            dev = to_dev(vals)
+            from parafaft_amd import _capi as A
+            A.check(A.load().pfx_memcpy_async(addr, dev.data_ptr(), count * vals.itemsize, None),
+                    "copy")
+            torch().cuda.synchronize()
+            peers[(t, p)] = vals
+    return peers
+
+
+def _oracle_share(real, share, peers):
+    """FP64 oracle pipeline for the share, with the same peer blocks."""
+    D = share.D
+    ins, outs = share.stage_shapes, share.stage_output_shapes
+    n = real.shape[-1]
+    h = n // 2 + 1
+    pad = np.zeros(real.shape[:-1] + (2 * h,))
+    pad[..., :n] = real
+    O.r2c_inplace(pad.reshape(-1, 2 * h), n, int(np.prod(real.shape[:-1])), 1, 2 * h)
+    src = pad.view(np.complex128).reshape(outs[0])
+    for t in range(D - 1):
+        g = share.geoms[t]
+        send = O.pack(src, g, 16).view(np.complex128)
+        recv = np.zeros(sum(g.recv_counts), np.complex128)
+        for p in range(len(g.recv_counts)):
+            seg = slice(g.recv_displs[p], g.recv_displs[p] + g.recv_counts[p])
+            if p == share.me[t]:
+                recv[seg] = send[g.send_displs[p]:g.send_displs[p] + g.send_counts[p]]
+            else:
+                recv[seg] = peers[(t, p)].astype(np.complex128)
+        dst = np.zeros(ins[t + 1], np.complex128)
+        O.unpack(recv.view(np.uint8), dst, g, 16)
+        axis = D - 2 - t
+        length = dst.shape[axis]
+        trail = int(np.prod(dst.shape[axis + 1:]))
+        lead = int(np.prod(dst.shape[:axis]))
+        flat = dst.reshape(lead, length * trail)
+        for i in range(lead):
+            row = np.ascontiguousarray(flat[i])
+            O.c2c_batched(row, length, trail, trail, 1, -1)
+            flat[i] = row
+        src = flat.reshape(ins[t + 1])
+    return src
+
+
+def _run_stepwise(share, real, rdt, check_bytes=True):
+    """Runs the share forward step by step on the GPU, checking every pack and
+    unpack bit-exact against the oracle loops on the GPU's own data."""
+    import parafaft_amd as P
+    from parafaft_amd import _capi as A
+    D = share.D
+    n = real.shape[-1]
+    h = n // 2 + 1
+    esz = share.esz
+    cdt = np.complex64 if esz == 8 else np.complex128
+    pad = np.zeros(real.shape[:-1] + (2 * h,), rdt)
+    pad[..., :n] = real
+    buf = np.zeros(share.get_required_output_size(), rdt)
+    buf[:pad.size] = pad.ravel()
+    d = to_dev(buf)
+    share.r2c.execute(A.PFX_FORWARD, d, None)
+    src, dst = d.data_ptr(), share.scratch.ptr
+    for t in range(D - 1):
+        g = share.geoms[t]
+        share.pack(t, src)
+        if check_bytes:
+            src_host = _download(src, math.prod(share.stage_output_shapes[t]) * esz)
+            ref = O.pack(src_host, g, esz)
+            got = _download(share.send[t].ptr, sum(g.send_counts) * esz)
+            assert np.array_equal(got, ref), f"pack transition {t} not bit-exact"
+        share.self_copy(t)
+        share.unpack(t, dst)
+        if check_bytes:
+            recv_host = _download(share.recv[t].ptr, sum(g.recv_counts) * esz)
+            ref = O.unpack(recv_host, np.zeros(math.prod(share.stage_shapes[t + 1]) * esz,
+                                               np.uint8), g, esz)
+            got = _download(dst, math.prod(share.stage_shapes[t + 1]) * esz)
+            assert np.array_equal(got, ref), f"unpack transition {t} not bit-exact"
+        share.fft_stage(t + 1, dst)
+        src, dst = dst, src
+    out = _download(src, math.prod(share.stage_shapes[D - 1]) * esz)
+    return out.view(cdt).reshape(share.stage_shapes[D - 1])
+
+
+def _download(addr, nbytes):
+    t = torch()
+    t.cuda.synchronize()
+    host = t.empty(nbytes, dtype=t.uint8)
+    from parafaft_amd import _capi as A
+    A.check(A.load().pfx_memcpy_async(host.data_ptr(), addr, nbytes, None), "d2h")
+    A.check(A.load().pfx_stream_synchronize(None), "sync")
+    return host.numpy()
+
+
+@pytest.mark.parametrize("coords", [(0, 0), (3, 1), (1, 2)])
+@pytest.mark.parametrize("fast", [True, False])
+def test_pencil_share_small_vs_oracle(coords, fast):
+    """128^3 real grid on a 4x4 grid: share 32 x 32 x 128 (complex last
+    axis 65 split 17/16/16/16, like 1025 in the 2048^3 case)."""
+    import parafaft_amd as P
+    from parafaft_amd.pencil import PencilShareR2C
+    G = [128, 128, 128]
+    for prec, rdt, cdt, tol in (("f32", np.float32, np.complex64, 1e-6 * 21),
+                                ("f64", np.float64, np.complex128, 1e-13)):
+        share = PencilShareR2C(G, [4, 4], list(coords), precision=prec, fast=fast)
+        real = filtered_noise(tuple(share.stage_shapes[0]), seed=31).astype(rdt)
+        peers = _fill_peers(share, 500, cdt)
+        got = _run_stepwise(share, real, rdt)
+        ref = _oracle_share(real.astype(np.float64), share, peers)
+        assert rel_l2(got, ref) <= tol, (prec, coords, rel_l2(got, ref))
+
+
+def test_geometry_matches_oracle():
+    from parafaft_amd import geometry as G
+    for gshape, dims, coords in [([2048] * 3, [4, 4], [0, 0]), ([2048] * 3, [4, 4], [3, 1]),
+                                 ([64, 48, 30], [2, 3], [1, 2]), ([16] * 4, [2, 2, 2], [1, 0, 1])]:
+        assert list(G.r2c_stage_shapes(gshape, dims, coords)) == \
+            list(O.stage_shapes_r2c(gshape, dims, coords))
+    g1 = G.exchange_geometry(4, [512, 512, 1025], 2, [512, 2048, 257], 1)
+    g2 = O.exchange_geometry(4, [512, 512, 1025], 2, [512, 2048, 257], 1)
+    assert g1.__dict__ == g2.__dict__
+
+
+def test_config3_full_share():
+    """Rank (0,0) of a 2048^3 R2C on a 4x4 grid: real share 512 x 512 x 2048
+    (padded 2050), half spectrum 512 x 512 x 1025; T0 and T1 pack/unpack into
+    4 + 4 device segments each, checked bit-exact on the GPU's own data; the
+    FP32 result against the FP64 GPU path on the same input (that path is
+    pinned to the oracle above at <= 1e-13)."""
+    from parafaft_amd.pencil import PencilShareR2C
+    G = [2048, 2048, 2048]
+    share32 = PencilShareR2C(G, [4, 4], [0, 0], precision="f32")
+    assert share32.stage_shapes == [[512, 512, 2048], [512, 2048, 257], [2048, 512, 257]]
+    assert len(share32.geoms) == 2 and all(len(g.send_counts) == 4 for g in share32.geoms)
+    rng = np.random.default_rng(77)
+    real = rng.standard_normal((512, 512, 2048)).astype(np.float32)
+    _fill_peers(share32, 900, np.complex64)
+    got32 = _run_stepwise(share32, real, np.float32, check_bytes=True)
+    del share32
+    share64 = PencilShareR2C(G, [4, 4], [0, 0], precision="f64")
+    _fill_peers(share64, 900, np.complex128)
+    got64 = _run_stepwise(share64, real.astype(np.float64), np.float64, check_bytes=False)
+    err = rel_l2(got32, got64)
+    assert err <= 1e-6 * 11, err
