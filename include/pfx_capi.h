@@ -196,6 +196,60 @@ int pfx_ndplan_destroy(pfx_ndplan plan);
 /* Stage plan of axis-stage s (0 = last axis), for per-stage timing. */
 int pfx_ndplan_stage(pfx_ndplan plan, int stage, pfx_plan *out);
 
+/* ------------------------------------------- many uneven transforms */
+
+/*
+ * One transform of a batch: `ndim` (1..4) axes of `shape`, in place at the
+ * device pointer `data`.  kind PFX_C2C: complex array; kind PFX_R2C: padded
+ * real array (last axis stored as 2*(shape[ndim-1]/2+1) scalars), forward =
+ * R2C, backward = C2R.
+ */
+typedef struct {
+  int kind;
+  int ndim;
+  int shape[4];
+  void *data;
+} pfx_batch_desc;
+
+typedef struct pfx_batch_t *pfx_batch;
+
+/*
+ * Plans `count` independent transforms of different shapes and kinds.  Their
+ * stage work is grouped by kernel variant and stage level, so one launch runs
+ * the same kernel over every transform that needs it (a device job table maps
+ * each workgroup to its transform and tile).  The data pointers are part of
+ * the plan.
+ * Replaces: one ParaFaFT_C2C / ParaFaFT_R2C object and plan set per shape,
+ *   each issuing its own launches (parafaft_c2c.hpp:539-694).
+ */
+int pfx_batch_create(pfx_batch *batch, int precision, int count, const pfx_batch_desc *descs);
+
+/* All transforms, grouped launches (stage level by stage level). */
+int pfx_batch_execute(pfx_batch batch, int direction, pfx_stream stream);
+
+/* All transforms, one launch per transform stage (the ungrouped baseline). */
+int pfx_batch_execute_each(pfx_batch batch, int direction, pfx_stream stream);
+
+/* Launch counts of the two schedules for one direction. */
+int pfx_batch_launches(pfx_batch batch, int direction, int *grouped, int *each);
+
+int pfx_batch_destroy(pfx_batch batch);
+
+/* --------------------------------------------------------- HIP graphs */
+
+typedef struct pfx_graph_t *pfx_graph;
+
+/*
+ * Capture every pfx call enqueued on `stream` (a non-null stream) between
+ * begin and end into an instantiated HIP graph; replay it with
+ * pfx_graph_launch.  Captured calls must not synchronise.  No library call
+ * records a memset node.
+ */
+int pfx_graph_begin_capture(pfx_stream stream);
+int pfx_graph_end_capture(pfx_stream stream, pfx_graph *graph);
+int pfx_graph_launch(pfx_graph graph, pfx_stream stream);
+int pfx_graph_destroy(pfx_graph graph);
+
 #ifdef __cplusplus
 }
 #endif

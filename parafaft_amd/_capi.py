@@ -67,7 +67,22 @@ SIGNATURES = [
     ("pfx_ndplan_execute", _i, [_vp, _i, _vp, _d, _vp]),
     ("pfx_ndplan_destroy", _i, [_vp]),
     ("pfx_ndplan_stage", _i, [_vp, _i, _vpp]),
+    ("pfx_batch_create", _i, [_vpp, _i, _i, _vp]),
+    ("pfx_batch_execute", _i, [_vp, _i, _vp]),
+    ("pfx_batch_execute_each", _i, [_vp, _i, _vp]),
+    ("pfx_batch_launches", _i, [_vp, _i, _ip, _ip]),
+    ("pfx_batch_destroy", _i, [_vp]),
+    ("pfx_graph_begin_capture", _i, [_vp]),
+    ("pfx_graph_end_capture", _i, [_vp, _vpp]),
+    ("pfx_graph_launch", _i, [_vp, _vp]),
+    ("pfx_graph_destroy", _i, [_vp]),
 ]
+
+
+class BatchDesc(C.Structure):
+    """pfx_batch_desc"""
+    _fields_ = [("kind", C.c_int), ("ndim", C.c_int), ("shape", C.c_int * 4),
+                ("data", C.c_void_p)]
 
 _lib = None
 

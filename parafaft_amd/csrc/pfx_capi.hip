@@ -538,4 +538,229 @@ int pfx_ndplan_stage(pfx_ndplan p, int stage, pfx_plan *out) {
   return PFX_SUCCESS;
 }
 
+// ------------------------------------------------- many uneven transforms
+
+}  // extern "C"
+
+namespace {
+
+struct Group {
+  const KernelEntry *kernel;
+  std::vector<StockhamArgs> jobs;
+  std::vector<int64_t> start;  // workgroup start of every job (+ total)
+  StockhamArgs *d_jobs = nullptr;
+  int64_t *d_start = nullptr;
+};
+
+struct Level {
+  std::vector<Group> groups;
+  std::vector<std::pair<pfx_plan, void *>> direct;  // stages without a tiled kernel
+};
+
+StockhamArgs job_args(pfx_plan p, void *data, int direction) {
+  const bool real = p->kind != PFX_C2C;
+  StockhamArgs a;
+  a.data = data;
+  a.tw = p->tw;
+  a.ptw = p->ptw;
+  a.total = p->inner * p->outer;
+  a.inner = p->inner;
+  a.stride = p->stride;
+  a.dist = real ? p->dist / 2 : p->dist;
+  a.dist_outer = real ? p->dist_outer / 2 : p->dist_outer;
+  a.scale = 1.0;
+  a.use_scale = 0;
+  a.backward = direction == PFX_BACKWARD;
+  a.xcd_remap = p->xcd_remap;
+  return a;
+}
+
+}  // namespace
+
+struct pfx_batch_t {
+  int precision = PFX_F32;
+  std::vector<pfx_ndplan> plans;
+  std::vector<void *> data;
+  std::vector<Level> levels[2];  // [direction][level]
+};
+
+struct pfx_graph_t {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+extern "C" {
+
+int pfx_batch_destroy(pfx_batch b) {
+  if (!b) return PFX_SUCCESS;
+  for (auto &dir : b->levels)
+    for (auto &lv : dir)
+      for (auto &g : lv.groups) {
+        if (g.d_jobs) hipFree(g.d_jobs);
+        if (g.d_start) hipFree(g.d_start);
+      }
+  for (pfx_ndplan p : b->plans) pfx_ndplan_destroy(p);
+  delete b;
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_desc *descs) {
+  if (!out) return fail(PFX_ERR_INVALID_ARG, "batch pointer is null");
+  *out = nullptr;
+  if (count < 0 || (count > 0 && !descs)) return fail(PFX_ERR_INVALID_ARG, "bad descriptor list");
+  pfx_batch b = new (std::nothrow) pfx_batch_t;
+  if (!b) return fail(PFX_ERR_ALLOC, "out of host memory");
+  b->precision = precision;
+  int maxd = 0;
+  for (int i = 0; i < count; ++i) {
+    const pfx_batch_desc &d = descs[i];
+    if (d.ndim < 1 || d.ndim > 4) {
+      pfx_batch_destroy(b);
+      return fail(PFX_ERR_INVALID_ARG, "transform %d: ndim must be 1..4", i);
+    }
+    pfx_ndplan nd = nullptr;
+    int rc = pfx_ndplan_create(&nd, precision, d.kind, d.ndim, d.shape);
+    if (rc != PFX_SUCCESS) {
+      std::string msg = g_last_error;
+      pfx_batch_destroy(b);
+      return fail(rc, "transform %d: %s", i, msg.c_str());
+    }
+    b->plans.push_back(nd);
+    b->data.push_back(d.data);
+    if (d.ndim > maxd) maxd = d.ndim;
+  }
+  for (int dir = 0; dir < 2; ++dir) {
+    b->levels[dir].resize(maxd);
+    for (int i = 0; i < count; ++i) {
+      pfx_ndplan nd = b->plans[i];
+      const int D = nd->ndim;
+      for (int k = 0; k < D; ++k) {
+        const int s = dir == PFX_FORWARD ? k : D - 1 - k;
+        pfx_plan pl = nd->stages[s];
+        if (nd->kind == PFX_R2C && s == 0 && dir == PFX_BACKWARD) pl = nd->stages[D];
+        Level &lv = b->levels[dir][k];
+        if (!pl->kernel) {
+          lv.direct.push_back({pl, b->data[i]});
+          continue;
+        }
+        Group *g = nullptr;
+        for (auto &gg : lv.groups)
+          if (gg.kernel == pl->kernel) g = &gg;
+        if (!g) {
+          lv.groups.push_back(Group{pl->kernel, {}, {}, nullptr, nullptr});
+          g = &lv.groups.back();
+        }
+        g->jobs.push_back(job_args(pl, b->data[i], dir));
+      }
+    }
+    // workgroup ranges, each starting at a multiple of 8 (XCD-aware order)
+    for (auto &lv : b->levels[dir])
+      for (auto &g : lv.groups) {
+        int64_t pos = 0;
+        for (const StockhamArgs &a : g.jobs) {
+          g.start.push_back(pos);
+          const int64_t tiles = (a.total + g.kernel->per_block - 1) / g.kernel->per_block;
+          pos += (tiles + 7) / 8 * 8;
+        }
+        g.start.push_back(pos);
+        if (pos > 0x7fffffff) {
+          pfx_batch_destroy(b);
+          return fail(PFX_ERR_UNSUPPORTED, "grouped launch exceeds the grid limit");
+        }
+        hipError_t e = hipMalloc(&g.d_jobs, g.jobs.size() * sizeof(StockhamArgs));
+        if (e == hipSuccess) e = hipMalloc(&g.d_start, g.start.size() * sizeof(int64_t));
+        if (e == hipSuccess)
+          e = hipMemcpy(g.d_jobs, g.jobs.data(), g.jobs.size() * sizeof(StockhamArgs),
+                        hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+          e = hipMemcpy(g.d_start, g.start.data(), g.start.size() * sizeof(int64_t),
+                        hipMemcpyHostToDevice);
+        if (e != hipSuccess) {
+          pfx_batch_destroy(b);
+          return hip_fail(e, "batch job tables");
+        }
+      }
+  }
+  *out = b;
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_execute(pfx_batch b, int direction, pfx_stream stream) {
+  if (!b) return fail(PFX_ERR_INVALID_ARG, "batch is null");
+  if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "bad direction");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  for (auto &lv : b->levels[direction]) {
+    for (auto &g : lv.groups) {
+      GroupArgs ga{g.d_jobs, g.d_start, (int)g.jobs.size()};
+      g.kernel->launch_group(ga, g.start.back(), s);
+    }
+    for (auto &dp : lv.direct) {
+      int rc = pfx_execute(dp.first, direction, dp.second, stream);
+      if (rc != PFX_SUCCESS) return rc;
+    }
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "grouped launch");
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_execute_each(pfx_batch b, int direction, pfx_stream stream) {
+  if (!b) return fail(PFX_ERR_INVALID_ARG, "batch is null");
+  for (size_t i = 0; i < b->plans.size(); ++i) {
+    int rc = pfx_ndplan_execute(b->plans[i], direction, b->data[i], 1.0, stream);
+    if (rc != PFX_SUCCESS) return rc;
+  }
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_launches(pfx_batch b, int direction, int *grouped, int *each) {
+  if (!b || direction < 0 || direction > 1) return fail(PFX_ERR_INVALID_ARG, "bad argument");
+  int g = 0, e = 0;
+  for (auto &lv : b->levels[direction]) g += (int)(lv.groups.size() + lv.direct.size());
+  for (pfx_ndplan p : b->plans) e += p->ndim;
+  if (grouped) *grouped = g;
+  if (each) *each = e;
+  return PFX_SUCCESS;
+}
+
+// ---------------------------------------------------------------- graphs
+
+int pfx_graph_begin_capture(pfx_stream stream) {
+  if (!stream) return fail(PFX_ERR_INVALID_ARG, "capture needs a non-null stream");
+  PFX_HIP(hipStreamBeginCapture(static_cast<hipStream_t>(stream), hipStreamCaptureModeThreadLocal),
+          "hipStreamBeginCapture");
+  return PFX_SUCCESS;
+}
+
+int pfx_graph_end_capture(pfx_stream stream, pfx_graph *out) {
+  if (!stream || !out) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  pfx_graph g = new (std::nothrow) pfx_graph_t;
+  if (!g) return fail(PFX_ERR_ALLOC, "out of host memory");
+  hipError_t e = hipStreamEndCapture(static_cast<hipStream_t>(stream), &g->graph);
+  if (e == hipSuccess) e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    if (g->graph) hipGraphDestroy(g->graph);
+    delete g;
+    return hip_fail(e, "graph capture");
+  }
+  *out = g;
+  return PFX_SUCCESS;
+}
+
+int pfx_graph_launch(pfx_graph g, pfx_stream stream) {
+  if (!g) return fail(PFX_ERR_INVALID_ARG, "graph is null");
+  PFX_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)), "hipGraphLaunch");
+  return PFX_SUCCESS;
+}
+
+int pfx_graph_destroy(pfx_graph g) {
+  if (!g) return PFX_SUCCESS;
+  if (g->exec) hipGraphExecDestroy(g->exec);
+  if (g->graph) hipGraphDestroy(g->graph);
+  delete g;
+  return PFX_SUCCESS;
+}
+
 }  // extern "C"

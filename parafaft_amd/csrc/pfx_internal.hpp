@@ -8,6 +8,7 @@
 namespace pfx {
 
 using LaunchFn = void (*)(const StockhamArgs &, hipStream_t);
+using GroupLaunchFn = void (*)(const GroupArgs &, int64_t blocks, hipStream_t);
 
 struct Radices {
   int n;
@@ -27,6 +28,7 @@ struct KernelEntry {
   const char *radices;
   LaunchFn launch;
   Radices rad;    // pass radices, first pass first
+  GroupLaunchFn launch_group;
 };
 
 template <int... Rs> constexpr int prod() { return (Rs * ... * 1); }
@@ -40,6 +42,22 @@ template <typename T, int TPT, int C, int... Rs>
 void launch_c2c_wide(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
   c2c_wide_kernel<T, TPT, C, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+}
+template <typename T, int TPT, int C, bool COL, int... Rs>
+void group_c2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
+  c2c_group_kernel<T, TPT, C, COL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
+}
+template <typename T, int TPT, int C, int... Rs>
+void group_c2c_wide(const GroupArgs &g, int64_t blocks, hipStream_t s) {
+  c2c_wide_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
+}
+template <typename T, int TPT, int C, int... Rs>
+void group_r2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
+  r2c_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
+}
+template <typename T, int TPT, int C, int... Rs>
+void group_c2r(const GroupArgs &g, int64_t blocks, hipStream_t s) {
+  c2r_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
 }
 template <typename T, int TPT, int C, int... Rs>
 void launch_r2c(const StockhamArgs &a, hipStream_t s) {

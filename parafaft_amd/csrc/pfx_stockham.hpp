@@ -209,9 +209,11 @@ __device__ inline int64_t xform_offset(int64_t b, const StockhamArgs& a) {
 // ---------------------------------------------------------------------------
 // C2C
 // ---------------------------------------------------------------------------
+// Kernel bodies take the workgroup's tile id and the tile count of its
+// launch (or of its job in a grouped launch), so the single-plan kernels and
+// the grouped multi-transform kernels share them.
 template <typename T, int TPT, int C, bool COL, int... Rs>
-__global__ void __launch_bounds__(TPT * C)
-c2c_kernel(StockhamArgs a) {
+__device__ inline void c2c_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, COL>;
   __shared__ V2<T> lds[L::size];
@@ -219,8 +221,7 @@ c2c_kernel(StockhamArgs a) {
   const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
-  int64_t tile = blockIdx.x;
-  if (COL && a.xcd_remap) tile = xcd_tile(tile, gridDim.x);
+  if (COL && a.xcd_remap) tile = xcd_tile(tile, ntiles);
   const int64_t b = tile * C + c;
   GlobalPtr<T> g;
   g.valid = b < a.total;
@@ -232,6 +233,11 @@ c2c_kernel(StockhamArgs a) {
   Passes<T, N, TPT, C, COL, 1, true, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
 }
 
+template <typename T, int TPT, int C, bool COL, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2c_kernel(StockhamArgs a) {
+  c2c_body<T, TPT, C, COL, Rs...>(a, blockIdx.x, gridDim.x);
+}
+
 // ---------------------------------------------------------------------------
 // R2C (forward): row of 2N reals = N complex z[n] = x[2n] + i x[2n+1];
 // Z = FFT_N(z); X[k] = (Z[k] + conj Z[N-k])/2 - (i/2) W_2N^k (Z[k] - conj Z[N-k]),
@@ -239,8 +245,7 @@ c2c_kernel(StockhamArgs a) {
 // Row layout only (stride 1).  a.dist is in complex units (= padded dist / 2).
 // ---------------------------------------------------------------------------
 template <typename T, int TPT, int C, int... Rs>
-__global__ void __launch_bounds__(TPT * C)
-r2c_kernel(StockhamArgs a) {
+__device__ inline void r2c_body(const StockhamArgs& a, int64_t tile) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
   __shared__ V2<T> lds[L::size];
@@ -248,7 +253,7 @@ r2c_kernel(StockhamArgs a) {
   const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
-  const int64_t b = (int64_t)blockIdx.x * C + c;
+  const int64_t b = tile * C + c;
   GlobalPtr<T> g;
   g.valid = b < a.total;
   g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
@@ -286,14 +291,18 @@ r2c_kernel(StockhamArgs a) {
   }
 }
 
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) r2c_kernel(StockhamArgs a) {
+  r2c_body<T, TPT, C, Rs...>(a, blockIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // C2R (backward): X[0..N] -> Z[k] = (X[k] + conj X[N-k]) + i W_2N^-k (X[k] - conj X[N-k]),
 // Im X[0] and Im X[N] ignored (Hermitian input); z = IFFT_N(Z) (unnormalised)
 // gives z[n] = x[2n] + i x[2n+1], stored as the N complex slots of the row.
 // ---------------------------------------------------------------------------
 template <typename T, int TPT, int C, int... Rs>
-__global__ void __launch_bounds__(TPT * C)
-c2r_kernel(StockhamArgs a) {
+__device__ inline void c2r_body(const StockhamArgs& a, int64_t tile) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
   __shared__ V2<T> lds[L::size];
@@ -301,7 +310,7 @@ c2r_kernel(StockhamArgs a) {
   const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
-  const int64_t b = (int64_t)blockIdx.x * C + c;
+  const int64_t b = tile * C + c;
   GlobalPtr<T> g;
   g.valid = b < a.total;
   g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
@@ -333,6 +342,11 @@ c2r_kernel(StockhamArgs a) {
   }
   __syncthreads();
   Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, ptw, g);
+}
+
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2r_kernel(StockhamArgs a) {
+  c2r_body<T, TPT, C, Rs...>(a, blockIdx.x);
 }
 
 }  // namespace pfx
@@ -422,8 +436,7 @@ struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
 };
 
 template <typename T, int TPT, int C, int... Rs>
-__global__ void __launch_bounds__(TPT * C)
-c2c_wide_kernel(StockhamArgs a) {
+__device__ inline void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   constexpr int E = N / TPT;
   constexpr int R0 = first_of<Rs...>();
@@ -433,8 +446,7 @@ c2c_wide_kernel(StockhamArgs a) {
   __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
   const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   const int c = threadIdx.x % C, j = threadIdx.x / C;
-  int64_t tile = blockIdx.x;
-  if (a.xcd_remap) tile = xcd_tile(tile, gridDim.x);
+  if (a.xcd_remap) tile = xcd_tile(tile, ntiles);
   const int64_t b = tile * C + c;
   GlobalPtr<T> g;
   g.valid = b < a.total;
@@ -455,6 +467,56 @@ c2c_wide_kernel(StockhamArgs a) {
     }
   }
   WidePasses<T, N, TPT, C, E, 1, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
+}
+
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2c_wide_kernel(StockhamArgs a) {
+  c2c_wide_body<T, TPT, C, Rs...>(a, blockIdx.x, gridDim.x);
+}
+
+// ---------------------------------------------------------------------------
+// Grouped launches: one launch runs the same kernel variant over many jobs
+// (independent transforms, or stages of them, with different geometry).  Job
+// j owns the workgroups [start[j], start[j+1]); every job's range begins at a
+// multiple of 8 so the XCD-aware tile order holds inside each job.
+// ---------------------------------------------------------------------------
+struct GroupArgs {
+  const StockhamArgs* jobs;
+  const int64_t* start;  // njobs + 1 entries
+  int njobs;
+};
+
+__device__ inline int find_job(const GroupArgs& ga, int64_t blk) {
+  int lo = 0, hi = ga.njobs - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ga.start[mid] <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename T, int TPT, int C, bool COL, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2c_group_kernel(GroupArgs ga) {
+  const int jb = find_job(ga, blockIdx.x);
+  const int64_t s0 = ga.start[jb];
+  c2c_body<T, TPT, C, COL, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
+}
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2c_wide_group_kernel(GroupArgs ga) {
+  const int jb = find_job(ga, blockIdx.x);
+  const int64_t s0 = ga.start[jb];
+  c2c_wide_body<T, TPT, C, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
+}
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) r2c_group_kernel(GroupArgs ga) {
+  const int jb = find_job(ga, blockIdx.x);
+  r2c_body<T, TPT, C, Rs...>(ga.jobs[jb], blockIdx.x - ga.start[jb]);
+}
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2r_group_kernel(GroupArgs ga) {
+  const int jb = find_job(ga, blockIdx.x);
+  c2r_body<T, TPT, C, Rs...>(ga.jobs[jb], blockIdx.x - ga.start[jb]);
 }
 
 }  // namespace pfx

@@ -225,3 +225,17 @@ def test_named_column_kernels(match, prec, n, monkeypatch):
         plan.execute(direction, d)
         ref = _ref_c2c(x, n, batch, batch, 1, sign)
         assert rel_l2(to_host(d), ref) <= tol(prec, n), (name, direction)
+
+
+def test_wide_4096_column_kernel(monkeypatch):
+    """The 4096-point 8-column wide variant (160 KiB LDS)."""
+    import parafaft_amd as P
+    monkeypatch.setenv("PFX_KERNEL_MATCH", "wide:16, 16, 16")
+    n, batch = 4096, 37
+    x = O.uniform_complex((n * batch,), seed=4096).astype(np.complex64)
+    plan = P.Plan.c2c("f32", n, batch, batch, 1)
+    assert "wide:16, 16, 16" in plan.describe()[0]
+    for direction, sign in ((0, -1), (1, +1)):
+        d = to_dev(x)
+        plan.execute(direction, d)
+        assert rel_l2(to_host(d), _ref_c2c(x, n, batch, batch, 1, sign)) <= tol("f32", n)
