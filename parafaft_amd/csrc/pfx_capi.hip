@@ -41,8 +41,9 @@ int hip_fail(hipError_t e, const char *what) {
 struct Registry {
   std::vector<const KernelEntry *> entries;
   Registry() {
-    const KernelEntry *(*fns[])(int *) = {table_f32_row, table_f32_col, table_f32_real,
-                                          table_f64_row, table_f64_col, table_f64_real};
+    const KernelEntry *(*fns[])(int *) = {table_f32_row,  table_f32_col, table_f32_real,
+                                          table_f64_row,  table_f64_col, table_f64_real,
+                                          table_split};
     for (auto fn : fns) {
       int n = 0;
       const KernelEntry *t = fn(&n);
@@ -138,8 +139,9 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
                        int64_t stride, int64_t dist, int64_t outer, int64_t dist_outer) {
   if (!out) return fail(PFX_ERR_INVALID_ARG, "plan pointer is null");
   *out = nullptr;
-  if (precision != PFX_F32 && precision != PFX_F64)
-    return fail(PFX_ERR_UNSUPPORTED, "precision %d has no tiled kernels yet", precision);
+  if (precision != PFX_F32 && precision != PFX_F64 && precision != PFX_F32_SPLIT_F16 &&
+      precision != PFX_F32_SPLIT_BF16)
+    return fail(PFX_ERR_UNSUPPORTED, "unknown precision %d", precision);
   if (length <= 0) return fail(PFX_ERR_INVALID_ARG, "length must be positive (got %lld)", (long long)length);
   if (inner < 0 || outer < 0) return fail(PFX_ERR_INVALID_ARG, "batch must be non-negative");
   if (stride <= 0) return fail(PFX_ERR_INVALID_ARG, "stride must be positive (got %lld)", (long long)stride);
@@ -175,7 +177,11 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   }
   // Batch-contiguous layout when consecutive transforms are adjacent.
   const bool col = (kind == PFX_C2C) && dist == 1 && stride > 1;
-  if (lds_n > 0) p->kernel = find_kernel(bp, kk, lds_n, col);
+  // Split precisions use their MFMA kernels where the plan has radix-16
+  // passes and the plain FP32 kernels elsewhere.
+  const bool split = precision == PFX_F32_SPLIT_F16 || precision == PFX_F32_SPLIT_BF16;
+  if (lds_n > 0 && split && kk == KIND_C2C) p->kernel = find_kernel(precision, kk, lds_n, col);
+  if (lds_n > 0 && !p->kernel) p->kernel = find_kernel(bp, kk, lds_n, col);
   if (!p->kernel) {
     if (length > naive_max_length(bp)) {
       delete p;
@@ -307,7 +313,8 @@ int pfx_plan_describe(pfx_plan p, char *name, size_t name_len, int *per_block, i
   char buf[256];
   if (p->kernel) {
     const KernelEntry *k = p->kernel;
-    snprintf(buf, sizeof(buf), "%s_%s_%s_n%d_r[%s]", k->prec ? "f64" : "f32",
+    static const char *pn[] = {"f32", "f64", "f32split_f16", "f32split_bf16"};
+    snprintf(buf, sizeof(buf), "%s_%s_%s_n%d_r[%s]", pn[k->prec & 3],
              k->kind == KIND_C2C ? "c2c" : (k->kind == KIND_R2C ? "r2c" : "c2r"),
              k->col ? "col" : "row", k->n, k->radices);
     if (per_block) *per_block = k->per_block;
@@ -639,7 +646,7 @@ int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_d
         pfx_plan pl = nd->stages[s];
         if (nd->kind == PFX_R2C && s == 0 && dir == PFX_BACKWARD) pl = nd->stages[D];
         Level &lv = b->levels[dir][k];
-        if (!pl->kernel) {
+        if (!pl->kernel || !pl->kernel->launch_group) {
           lv.direct.push_back({pl, b->data[i]});
           continue;
         }

@@ -17,7 +17,7 @@ struct Radices {
 template <int... Rs> constexpr Radices radices() { return Radices{(int)sizeof...(Rs), {Rs...}}; }
 
 struct KernelEntry {
-  int prec;       // PFX_F32 / PFX_F64
+  int prec;       // PFX_F32 / PFX_F64 / PFX_F32_SPLIT_F16 / PFX_F32_SPLIT_BF16
   int kind;       // KIND_C2C / KIND_R2C / KIND_C2R
   int n;          // complex FFT length held in LDS (R2C/C2R: half the real length)
   int col;        // 1: batch-contiguous layout, 0: row layout
@@ -28,7 +28,7 @@ struct KernelEntry {
   const char *radices;
   LaunchFn launch;
   Radices rad;    // pass radices, first pass first
-  GroupLaunchFn launch_group;
+  GroupLaunchFn launch_group;  // null: no grouped form
 };
 
 template <int... Rs> constexpr int prod() { return (Rs * ... * 1); }
@@ -81,6 +81,7 @@ const KernelEntry *table_f32_real(int *count);
 const KernelEntry *table_f64_row(int *count);
 const KernelEntry *table_f64_col(int *count);
 const KernelEntry *table_f64_real(int *count);
+const KernelEntry *table_split(int *count);
 
 // Fallback kernels (any length that fits LDS): direct O(n^2) DFT.
 struct NaiveArgs {

@@ -1,0 +1,69 @@
+"""Split-precision MFMA form: radix-16 sub-DFTs on v_mfma_f32_16x16x16 with
+FP16 hi/lo (per-group power-of-two scaling) or BF16 hi/mid/lo operand splits,
+FP32 accumulation; every result against the FP64 oracle, at the FP32 bound
+rel-L2 <= 1e-6 * log2(N)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import rel_l2, to_dev, to_host, tol
+
+pytestmark = pytest.mark.gpu
+
+MODES = ["f32_split_f16", "f32_split_bf16"]
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n", [256, 1024, 2048, 4096])
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_split_1d(mode, n, layout):
+    import parafaft_amd as P
+    batch = 33
+    x = O.uniform_complex((n * batch,), seed=n + len(mode)).astype(np.complex64)
+    if layout == "row":
+        stride, dist = 1, n
+    else:
+        stride, dist = batch, 1
+    plan = P.Plan.c2c(mode, n, batch, stride, dist)
+    name = plan.describe()[0]
+    assert "split" in name, name
+    for direction, sign in ((0, -1), (1, +1)):
+        d = to_dev(x)
+        plan.execute(direction, d)
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, stride, dist, sign)
+        err = rel_l2(to_host(d), ref)
+        assert err <= tol("f32", n), (name, direction, err)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_split_wide_dynamic_range(mode):
+    """Inputs spanning 2^-20 .. 2^20 within one transform (the FP16 form's
+    per-group scaling keeps the large values finite)."""
+    import parafaft_amd as P
+    n, batch = 1024, 8
+    rng = np.random.default_rng(5)
+    mag = 2.0 ** rng.uniform(-20, 20, n * batch)
+    x = (mag * np.exp(2j * np.pi * rng.random(n * batch))).astype(np.complex64)
+    plan = P.Plan.c2c(mode, n, batch, 1, n)
+    d = to_dev(x)
+    plan.execute(0, d)
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, 1, n, -1)
+    got = to_host(d)
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= tol("f32", n)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_split_2d_through_fast_path(mode):
+    import parafaft_amd as P
+    n = 1024
+    x = O.uniform_complex((256, n), seed=3).astype(np.complex64)
+    fft = P.ParaFaFT_C2C([256, n], precision=mode, fast=True)
+    d = to_dev(x)
+    fft.forward(d)
+    ref = O.fftn(x.astype(np.complex128))
+    assert rel_l2(to_host(d), ref) <= tol("f32", 256 * n)
+    fft.backward(d, scale=1.0 / (256 * n))
+    assert rel_l2(to_host(d), x) <= tol("f32", 256 * n)
