@@ -110,9 +110,15 @@ public:
 
   static constexpr int precision = std::is_same<FloatTypeT, double>::value ? PFX_F64 : SplitMode;
 
+  /// plan_flag: Estimate picks kernels from a static table; Measure / Patient
+  /// time the candidate kernels on the planning buffer the driver passes to
+  /// create_stage_plan (its scratch buffer, whose contents FFTW_MEASURE also
+  /// destroys) and keep the fastest (pfx_plan_tune, cached per geometry).
   explicit MI355Backend(int num_stages, FFTPlanFlag plan_flag = FFTPlanFlag::Estimate)
-      : num_stages_(num_stages), plans_(num_stages, nullptr) {
-    (void)plan_flag;
+      : num_stages_(num_stages), plans_(num_stages, nullptr),
+        tune_flag_(plan_flag == FFTPlanFlag::Estimate
+                       ? PFX_PLAN_ESTIMATE
+                       : (plan_flag == FFTPlanFlag::Measure ? PFX_PLAN_MEASURE : PFX_PLAN_PATIENT)) {
     check_pfx(pfx_stream_create(&stream_), "pfx_stream_create");
   }
 
@@ -140,7 +146,7 @@ public:
 
   MI355Backend(MI355Backend &&o) noexcept
       : num_stages_(o.num_stages_), plans_(std::move(o.plans_)), r2c_(o.r2c_), c2r_(o.c2r_),
-        stream_(o.stream_) {
+        stream_(o.stream_), tune_flag_(o.tune_flag_) {
     o.plans_.assign(o.num_stages_, nullptr);
     o.r2c_ = o.c2r_ = nullptr;
     o.stream_ = nullptr;
@@ -150,11 +156,11 @@ public:
   /// fft_backend_hipfft.hpp:294-314; `data` is a layout reference only.
   void create_stage_plan(int stage, int length, std::size_t batch, Complex *data,
                          std::ptrdiff_t stride, std::ptrdiff_t dist) {
-    (void)data;
     check_stage(stage);
     pfx_plan p = nullptr;
     check_pfx(pfx_plan_create_c2c(&p, precision, length, checked(batch, "batch"), stride, dist),
               "pfx_plan_create_c2c");
+    tune(p, data);
     pfx_plan_destroy(plans_[stage]);
     plans_[stage] = p;
   }
@@ -176,10 +182,10 @@ public:
   /// fft_backend_hipfft.hpp:361-384 (dist in FloatType scalars).
   void create_r2c_inplace_plan(int length, std::size_t batch, FloatType *padded_real,
                                std::ptrdiff_t stride, std::ptrdiff_t dist) {
-    (void)padded_real;
     pfx_plan p = nullptr;
     check_pfx(pfx_plan_create_r2c(&p, precision, length, checked(batch, "batch"), stride, dist),
               "pfx_plan_create_r2c");
+    tune(p, padded_real);
     pfx_plan_destroy(r2c_);
     r2c_ = p;
   }
@@ -191,10 +197,10 @@ public:
   /// fft_backend_hipfft.hpp:426-447.
   void create_c2r_inplace_plan(int length, std::size_t batch, FloatType *padded_real,
                                std::ptrdiff_t stride, std::ptrdiff_t dist) {
-    (void)padded_real;
     pfx_plan p = nullptr;
     check_pfx(pfx_plan_create_c2r(&p, precision, length, checked(batch, "batch"), stride, dist),
               "pfx_plan_create_c2r");
+    tune(p, padded_real);
     pfx_plan_destroy(c2r_);
     c2r_ = p;
   }
@@ -250,6 +256,11 @@ public:
   pfx_stream stream() const { return stream_; }
 
 private:
+  void tune(pfx_plan p, void *planning_buffer) {
+    if (tune_flag_ == PFX_PLAN_ESTIMATE || planning_buffer == nullptr) return;
+    check_pfx(pfx_plan_tune(p, tune_flag_, planning_buffer, stream_), "pfx_plan_tune");
+  }
+
   static int dir(FFTDirection d) { return d == FFTDirection::Forward ? PFX_FORWARD : PFX_BACKWARD; }
 
   void check_stage(int stage) const {
@@ -269,6 +280,7 @@ private:
   pfx_plan r2c_ = nullptr;
   pfx_plan c2r_ = nullptr;
   pfx_stream stream_ = nullptr;
+  int tune_flag_ = PFX_PLAN_ESTIMATE;
 };
 
 }  // namespace parafaft

@@ -113,6 +113,36 @@ int pfx_execute_scaled(pfx_plan plan, int direction, void *data, double scale,
 
 int pfx_plan_destroy(pfx_plan plan);
 
+/*
+ * Planning effort (the FFTPlanFlag of backend/fft_backend.hpp:39-43).
+ * PFX_PLAN_ESTIMATE: a static priority table picks the kernel.
+ * PFX_PLAN_MEASURE / PFX_PLAN_PATIENT: pfx_plan_tune times every candidate
+ * kernel of the plan (PATIENT with more repetitions) and keeps the fastest;
+ * results go to a process-wide tuning cache keyed by the plan geometry, which
+ * later plans of the same key reuse without timing.
+ */
+enum pfx_plan_flag { PFX_PLAN_ESTIMATE = 0, PFX_PLAN_MEASURE = 1, PFX_PLAN_PATIENT = 2 };
+
+/*
+ * Times the candidate kernels of `plan` on `scratch` (a device buffer with the
+ * plan's layout; its contents are destroyed, as FFTW_MEASURE destroys the
+ * planning array) and switches the plan to the fastest.  A cache hit returns
+ * without timing.  flag PFX_PLAN_ESTIMATE is a no-op.
+ * Replaces: FFTW_MEASURE / FFTW_PATIENT planning in FFTWBackend::create_stage_plan
+ *   (backend/fftw3/fft_backend_fftw.hpp:387-424, 743-754).
+ */
+int pfx_plan_tune(pfx_plan plan, int flag, void *scratch, pfx_stream stream);
+
+/*
+ * Saves / loads / clears the tuning cache (text, one entry per line).
+ * Replaces: FFTW wisdom export / import (backend/fftw3/fft_backend_fftw.hpp:666-694).
+ */
+int pfx_tune_cache_save(const char *path);
+int pfx_tune_cache_load(const char *path);
+int pfx_tune_cache_clear(void);
+/* Number of entries in the tuning cache. */
+int pfx_tune_cache_size(void);
+
 /* Kernel chosen for a plan: name buffer, transforms per workgroup, threads
  * per workgroup.  Diagnostics only. */
 int pfx_plan_describe(pfx_plan plan, char *name, size_t name_len, int *per_block,
@@ -192,6 +222,10 @@ int pfx_ndplan_create(pfx_ndplan *plan, int precision, int kind, int ndim, const
 int pfx_ndplan_execute(pfx_ndplan plan, int direction, void *data, double scale,
                        pfx_stream stream);
 int pfx_ndplan_destroy(pfx_ndplan plan);
+
+/* Tunes every stage plan (pfx_plan_tune) on `scratch`, a device buffer of the
+ * whole array (contents destroyed). */
+int pfx_ndplan_tune(pfx_ndplan plan, int flag, void *scratch, pfx_stream stream);
 
 /* Stage plan of axis-stage s (0 = last axis), for per-stage timing. */
 int pfx_ndplan_stage(pfx_ndplan plan, int stage, pfx_plan *out);

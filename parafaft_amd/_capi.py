@@ -30,6 +30,10 @@ PFX_C2R = 2
 PFX_FORWARD = 0
 PFX_BACKWARD = 1
 
+PFX_PLAN_ESTIMATE = 0
+PFX_PLAN_MEASURE = 1
+PFX_PLAN_PATIENT = 2
+
 # (name, restype, argtypes) for every symbol include/pfx_capi.h declares.
 _i = C.c_int
 _i64 = C.c_int64
@@ -53,6 +57,11 @@ SIGNATURES = [
     ("pfx_execute_scaled", _i, [_vp, _i, _vp, _d, _vp]),
     ("pfx_plan_destroy", _i, [_vp]),
     ("pfx_plan_describe", _i, [_vp, C.c_char_p, _sz, _ip, _ip]),
+    ("pfx_plan_tune", _i, [_vp, _i, _vp, _vp]),
+    ("pfx_tune_cache_save", _i, [C.c_char_p]),
+    ("pfx_tune_cache_load", _i, [C.c_char_p]),
+    ("pfx_tune_cache_clear", _i, []),
+    ("pfx_tune_cache_size", _i, []),
     ("pfx_malloc", _i, [_vpp, _sz]),
     ("pfx_free", _i, [_vp]),
     ("pfx_memcpy_async", _i, [_vp, _vp, _sz, _vp]),
@@ -67,6 +76,7 @@ SIGNATURES = [
     ("pfx_ndplan_execute", _i, [_vp, _i, _vp, _d, _vp]),
     ("pfx_ndplan_destroy", _i, [_vp]),
     ("pfx_ndplan_stage", _i, [_vp, _i, _vpp]),
+    ("pfx_ndplan_tune", _i, [_vp, _i, _vp, _vp]),
     ("pfx_batch_create", _i, [_vpp, _i, _i, _vp]),
     ("pfx_batch_execute", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute_each", _i, [_vp, _i, _vp]),

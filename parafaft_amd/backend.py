@@ -25,7 +25,9 @@ class FFTDirection(enum.Enum):
 
 
 class FFTPlanFlag(enum.Enum):
-    """backend/fft_backend.hpp:39-43 (accepted, as the GPU backends do)."""
+    """backend/fft_backend.hpp:39-43.  Estimate: static kernel table; Measure /
+    Patient: the candidate kernels are timed on the planning buffer
+    (pfx_plan_tune) and the fastest is kept and cached."""
     Estimate = 0
     Measure = 1
     Patient = 2
@@ -45,6 +47,22 @@ def ptr(x) -> int:
 
 _PREC = {"f32": A.PFX_F32, "float": A.PFX_F32, "f64": A.PFX_F64, "double": A.PFX_F64,
          "f32_split_f16": A.PFX_F32_SPLIT_F16, "f32_split_bf16": A.PFX_F32_SPLIT_BF16}
+
+
+def tune_cache_save(path: str):
+    A.check(A.load().pfx_tune_cache_save(path.encode()), "pfx_tune_cache_save")
+
+
+def tune_cache_load(path: str):
+    A.check(A.load().pfx_tune_cache_load(path.encode()), "pfx_tune_cache_load")
+
+
+def tune_cache_clear():
+    A.load().pfx_tune_cache_clear()
+
+
+def tune_cache_size() -> int:
+    return A.load().pfx_tune_cache_size()
 
 
 def precision_code(p) -> int:
@@ -96,6 +114,12 @@ class Plan:
             A.check(L.pfx_execute_scaled(self.handle, direction, ptr(data), scale, ptr(stream)),
                     "pfx_execute_scaled")
 
+    def tune(self, flag: int, scratch, stream=None):
+        """Time the candidate kernels on `scratch` (destroyed) and keep the
+        fastest (FFTW_MEASURE / FFTW_PATIENT semantics)."""
+        A.check(A.load().pfx_plan_tune(self.handle, int(flag), ptr(scratch), ptr(stream)),
+                "pfx_plan_tune")
+
     def describe(self):
         L = A.load()
         buf = C.create_string_buffer(256)
@@ -121,7 +145,7 @@ class MI355Backend:
 
     use_alltoallw = False        # drivers pack on the device (c2c:219-221, 576-580)
     handles_distributed = False  # the per-stage driver loop stays in charge
-    use_p2p = False              # one GPU: no IPC peers
+    use_p2p = True               # P2P statics present; no peer reports access
 
     def __init__(self, num_stages: int, plan_flag: FFTPlanFlag = FFTPlanFlag.Estimate,
                  precision="f64", stream=None):
@@ -146,6 +170,7 @@ class MI355Backend:
         """fft_backend_hipfft.hpp:294-314 (data is a layout reference only)."""
         old = self._plans[stage]
         self._plans[stage] = Plan.c2c(self.precision, length, batch, stride, dist)
+        self._tune(self._plans[stage], data)
         if old is not None:
             old.destroy()
 
@@ -155,14 +180,20 @@ class MI355Backend:
         self._plans[stage].execute(d, data, self.stream, scale)
 
     # -- R2C / C2R ------------------------------------------------------------
+    def _tune(self, plan, data):
+        if self.plan_flag != FFTPlanFlag.Estimate and ptr(data):
+            plan.tune(self.plan_flag.value, data, self.stream)
+
     def create_r2c_inplace_plan(self, length, batch, padded_real, stride, dist):
         self._r2c = Plan.r2c(self.precision, length, batch, stride, dist)
+        self._tune(self._r2c, padded_real)
 
     def execute_r2c_inplace(self, padded_real, scale: float = 1.0):
         self._r2c.execute(A.PFX_FORWARD, padded_real, self.stream, scale)
 
     def create_c2r_inplace_plan(self, length, batch, padded_real, stride, dist):
         self._c2r = Plan.c2r(self.precision, length, batch, stride, dist)
+        self._tune(self._c2r, padded_real)
 
     def execute_c2r_inplace(self, padded_real, scale: float = 1.0):
         self._c2r.execute(A.PFX_BACKWARD, padded_real, self.stream, scale)

@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -109,6 +110,34 @@ struct pfx_plan_t {
   int64_t tw_len = 0;
 };
 
+namespace {
+
+// Tuning cache: plan geometry -> chosen kernel (precision code, radix
+// description and workgroup tile identify it uniquely in the registry).
+std::mutex g_tune_mu;
+std::map<std::string, std::string> g_tune;
+
+std::string entry_id(const KernelEntry *k) {
+  return std::to_string(k->prec) + "|" + std::to_string(k->kind) + "|" + std::to_string(k->n) +
+         "|" + std::to_string(k->col) + "|" + std::to_string(k->per_block) + "|" + k->radices;
+}
+
+const KernelEntry *entry_by_id(const std::string &id) {
+  for (const KernelEntry *e : registry().entries)
+    if (entry_id(e) == id) return e;
+  return nullptr;
+}
+
+std::string plan_key(const pfx_plan_t *p) {
+  char buf[256];
+  snprintf(buf, sizeof(buf), "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld", p->precision, p->kind,
+           (long long)p->length, (long long)p->inner, (long long)p->outer, (long long)p->stride,
+           (long long)p->dist, (long long)p->dist_outer);
+  return buf;
+}
+
+}  // namespace
+
 struct pfx_ndplan_t {
   int precision = PFX_F32;
   int kind = PFX_C2C;
@@ -182,6 +211,12 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   const bool split = precision == PFX_F32_SPLIT_F16 || precision == PFX_F32_SPLIT_BF16;
   if (lds_n > 0 && split && kk == KIND_C2C) p->kernel = find_kernel(precision, kk, lds_n, col);
   if (lds_n > 0 && !p->kernel) p->kernel = find_kernel(bp, kk, lds_n, col);
+  if (p->kernel) {  // a tuned choice for this geometry wins over the table
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(plan_key(p));
+    if (it != g_tune.end())
+      if (const KernelEntry *e = entry_by_id(it->second)) p->kernel = e;
+  }
   if (!p->kernel) {
     if (length > naive_max_length(bp)) {
       delete p;
@@ -222,6 +257,37 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   }
   *out = p;
   return PFX_SUCCESS;
+}
+
+// Candidates for a plan: every registry entry that computes the same
+// transform (same kind and LDS length, same data precision family, same layout).
+static std::vector<const KernelEntry *> candidates(const pfx_plan_t *p) {
+  std::vector<const KernelEntry *> out;
+  if (!p->kernel) return out;
+  const KernelEntry *k0 = p->kernel;
+  for (const KernelEntry *e : registry().entries) {
+    if (e->kind != k0->kind || e->n != k0->n || e->col != k0->col) continue;
+    if (e->prec != k0->prec) continue;  // same arithmetic (plain or the same split form)
+    out.push_back(e);
+  }
+  return out;
+}
+
+static hipError_t set_kernel(pfx_plan_t *p, const KernelEntry *k) {
+  if (k == p->kernel) return hipSuccess;
+  const size_t esz = base_prec(p->precision) ? 16 : 8;
+  void *ptw = nullptr;
+  hipError_t e = hipMalloc(&ptw, (size_t)k->n * esz);
+  if (e == hipSuccess) e = make_pass_twiddles(base_prec(p->precision), k->rad, ptw, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    if (ptw) hipFree(ptw);
+    return e;
+  }
+  if (p->ptw) hipFree(p->ptw);
+  p->ptw = ptw;
+  p->kernel = k;
+  return hipSuccess;
 }
 
 int pfx_plan_create_c2c(pfx_plan *plan, int precision, int64_t length, int64_t batch,
@@ -298,6 +364,99 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
 
 int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream) {
   return pfx_execute_scaled(plan, direction, data, 1.0, stream);
+}
+
+int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (flag == PFX_PLAN_ESTIMATE || !p->kernel) return PFX_SUCCESS;
+  if (flag != PFX_PLAN_MEASURE && flag != PFX_PLAN_PATIENT)
+    return fail(PFX_ERR_INVALID_ARG, "unknown planning flag %d", flag);
+  if (!scratch) return fail(PFX_ERR_INVALID_ARG, "tuning needs a scratch buffer");
+  const std::string key = plan_key(p);
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune.find(key);
+    if (it != g_tune.end()) {
+      if (const KernelEntry *e = entry_by_id(it->second)) {
+        PFX_HIP(set_kernel(p, e), "tuned kernel");
+        return PFX_SUCCESS;
+      }
+    }
+  }
+  const int reps = flag == PFX_PLAN_PATIENT ? 10 : 3;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t a, b;
+  PFX_HIP(hipEventCreate(&a), "hipEventCreate");
+  PFX_HIP(hipEventCreate(&b), "hipEventCreate");
+  const KernelEntry *best = p->kernel;
+  float best_ms = -1.f;
+  const int dir = p->kind == PFX_C2R ? PFX_BACKWARD : PFX_FORWARD;
+  for (const KernelEntry *k : candidates(p)) {
+    hipError_t e = set_kernel(p, k);
+    if (e != hipSuccess) continue;
+    int rc = pfx_execute(p, dir, scratch, stream);  // warm-up
+    if (rc != PFX_SUCCESS) continue;
+    hipEventRecord(a, s);
+    for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r) rc = pfx_execute(p, dir, scratch, stream);
+    hipEventRecord(b, s);
+    if (hipEventSynchronize(b) != hipSuccess || rc != PFX_SUCCESS) continue;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, a, b);
+    if (best_ms < 0.f || ms < best_ms) {
+      best_ms = ms;
+      best = k;
+    }
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  PFX_HIP(set_kernel(p, best), "tuned kernel");
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune[key] = entry_id(best);
+  return PFX_SUCCESS;
+}
+
+int pfx_tune_cache_save(const char *path) {
+  if (!path) return fail(PFX_ERR_INVALID_ARG, "path is null");
+  FILE *f = fopen(path, "w");
+  if (!f) return fail(PFX_ERR_INVALID_ARG, "cannot write %s", path);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  fprintf(f, "# pfx tuning cache v1 (gfx950)\n");
+  for (auto &kv : g_tune) fprintf(f, "%s\t%s\n", kv.first.c_str(), kv.second.c_str());
+  fclose(f);
+  return PFX_SUCCESS;
+}
+
+int pfx_tune_cache_load(const char *path) {
+  if (!path) return fail(PFX_ERR_INVALID_ARG, "path is null");
+  FILE *f = fopen(path, "r");
+  if (!f) return fail(PFX_ERR_INVALID_ARG, "cannot read %s", path);
+  char line[1024];
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  int bad = 0;
+  while (fgets(line, sizeof(line), f)) {
+    if (line[0] == '#' || line[0] == '\n') continue;
+    char *tab = strchr(line, '\t');
+    if (!tab) { ++bad; continue; }
+    *tab = 0;
+    std::string val(tab + 1);
+    while (!val.empty() && (val.back() == '\n' || val.back() == '\r')) val.pop_back();
+    if (!entry_by_id(val)) { ++bad; continue; }  // kernel no longer in this build
+    g_tune[line] = val;
+  }
+  fclose(f);
+  (void)bad;
+  return PFX_SUCCESS;
+}
+
+int pfx_tune_cache_clear(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune.clear();
+  return PFX_SUCCESS;
+}
+
+int pfx_tune_cache_size(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return (int)g_tune.size();
 }
 
 int pfx_plan_destroy(pfx_plan p) {
@@ -535,6 +694,15 @@ int pfx_ndplan_destroy(pfx_ndplan p) {
   if (!p) return PFX_SUCCESS;
   for (pfx_plan s : p->stages) pfx_plan_destroy(s);
   delete p;
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  for (pfx_plan s : p->stages) {
+    int rc = pfx_plan_tune(s, flag, scratch, stream);
+    if (rc != PFX_SUCCESS) return rc;
+  }
   return PFX_SUCCESS;
 }
 

@@ -129,6 +129,28 @@ template <typename T> static void test_multiple_stages() {
   check(max_err(h, orig, 1.0 / tot) < tol<T>(), "3 stages roundtrip", max_err(h, orig, 1.0 / tot));
 }
 
+// Test 3b: Measure planning (candidate kernels timed on the planning buffer)
+static void test_measure_planning() {
+  const int n0 = 8, n1 = 1024, n2 = 64, tot = n0 * n1 * n2;
+  std::vector<double> u(2 * tot);
+  orc_uniform(u.data(), u.size(), 78);
+  std::vector<std::complex<float>> h(tot);
+  for (int i = 0; i < tot; ++i) h[i] = std::complex<float>(float(u[2 * i]), float(u[2 * i + 1]));
+  std::vector<std::complex<double>> ref(h.begin(), h.end());
+  parafaft::MI355Backend<float> be(1, parafaft::FFTPlanFlag::Measure);
+  typename parafaft::MI355Backend<float>::ComplexBuffer scratch(tot), d(tot);
+  const int before = pfx_tune_cache_size();
+  be.create_stage_plan(0, n1, n2, scratch.data(), n2, 1);  // middle-axis geometry, tuned
+  h2d(d.data(), h);
+  for (int i = 0; i < n0; ++i) be.execute_stage(0, FFTDirection::Forward, d.data() + i * n1 * n2);
+  be.sync();
+  d2h(h, d.data());
+  for (int i = 0; i < n0; ++i)
+    orc_c2c_batched(reinterpret_cast<double *>(ref.data() + i * n1 * n2), n1, n2, n2, 1, -1, 1);
+  double e = max_err(h, ref);
+  check(e < 2e-5 * n1 && pfx_tune_cache_size() == before + 1, "Measure planning: tuned + correct", e);
+}
+
 // Test 4: one plan applied to different pointers (new-array execute)
 template <typename T> static void test_different_pointers() {
   const int N = 64, batch = 2;
@@ -231,6 +253,7 @@ int main() {
   test_multiple_stages<float>();
   test_different_pointers<float>();
   test_r2c_c2r<float>();
+  test_measure_planning();
   test_errors();
   std::printf("%d failure(s)\n", failures);
   return failures;

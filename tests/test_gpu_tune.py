@@ -1,0 +1,55 @@
+"""Measure/Patient planning (pfx_plan_tune) and the tuning cache (the FFTW
+wisdom analogue, backend/fftw3/fft_backend_fftw.hpp:666-694)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import rel_l2, to_dev, to_host, tol
+
+pytestmark = pytest.mark.gpu
+
+
+def test_tune_picks_a_correct_kernel_and_caches(tmp_path):
+    import parafaft_amd as P
+    from parafaft_amd import backend as B
+    B.tune_cache_clear()
+    n, batch = 1024, 4096
+    x = O.uniform_complex((n * batch,), seed=17).astype(np.complex64)
+    scratch = to_dev(np.zeros_like(x))
+    plan = P.Plan.c2c("f32", n, batch, batch, 1)
+    default_name = plan.describe()[0]
+    plan.tune(1, scratch)
+    tuned = plan.describe()[0]
+    assert B.tune_cache_size() == 1
+    d = to_dev(x)
+    plan.execute(0, d)
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, batch, 1, -1)
+    assert rel_l2(to_host(d), ref) <= tol("f32", n), (default_name, tuned)
+    # persist, clear, reload: a new plan of the same geometry starts tuned
+    path = str(tmp_path / "pfx_tune.txt")
+    B.tune_cache_save(path)
+    assert open(path).read().count("\n") >= 2
+    B.tune_cache_clear()
+    assert B.tune_cache_size() == 0
+    B.tune_cache_load(path)
+    assert B.tune_cache_size() == 1
+    again = P.Plan.c2c("f32", n, batch, batch, 1)
+    assert again.describe()[0] == tuned
+    B.tune_cache_clear()
+
+
+def test_backend_measure_flag_through_driver():
+    import parafaft_amd as P
+    from parafaft_amd import backend as B
+    B.tune_cache_clear()
+    n = 64
+    x = O.uniform_complex((n, n, n), seed=2).astype(np.complex64)
+    fft = P.ParaFaFT_C2C([n] * 3, precision="f32", plan_flag=P.FFTPlanFlag.Patient)
+    assert B.tune_cache_size() == 3
+    d = to_dev(x)
+    fft.forward(d)
+    assert rel_l2(to_host(d), O.fftn(x.astype(np.complex128))) <= tol("f32", n ** 3)
+    B.tune_cache_clear()
