@@ -208,6 +208,9 @@ def main():
     ap.add_argument("--precision", default="f32")
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--plan", default="measure", choices=["estimate", "measure", "patient"],
+                    help="planning effort (FFTPlanFlag); measure/patient time the candidate "
+                         "kernels on the data buffer before the input is generated")
     ap.add_argument("--workload", default="cube", choices=["cube", "pencil", "batch"],
                     help="cube: the 1024^3 C2C headline; pencil: one rank's share of a "
                          "2048^3 R2C on a 4x4 grid with on-device pack/unpack")
@@ -264,6 +267,16 @@ def main():
     nd = C.c_void_p()
     shape = (C.c_int * 3)(n, n, n)
     A.check(L.pfx_ndplan_create(C.byref(nd), prec, A.PFX_C2C, 3, shape), "pfx_ndplan_create")
+    flag = {"estimate": A.PFX_PLAN_ESTIMATE, "measure": A.PFX_PLAN_MEASURE,
+            "patient": A.PFX_PLAN_PATIENT}[args.plan]
+    if flag != A.PFX_PLAN_ESTIMATE:
+        log(f"[bench] planning ({args.plan}) on the data buffer")
+        A.check(L.pfx_ndplan_tune(nd.value, flag, x.data_ptr(), sp), "pfx_ndplan_tune")
+        torch.cuda.synchronize()
+        g.manual_seed(20241203)   # planning destroyed the buffer: regenerate the input
+        xr = torch.view_as_real(x.view(-1))
+        xr.copy_(torch.rand(xr.shape, device=dev, generator=g, dtype=rdt) * 2 - 1)
+        torch.cuda.synchronize()
     stages = []
     for s in range(D):
         h = C.c_void_p()
@@ -369,7 +382,7 @@ def main():
         "data": "synthetic: iid Uniform[-1,1) re/im (seeded), resident in HBM",
         "config": {"workload": f"3D C2C forward, in place, {n}^3 complex<float>, one GPU "
                                "(one-rank fast path: one launch per axis)",
-                   "grid": [n, n, n], "precision": args.precision,
+                   "grid": [n, n, n], "precision": args.precision, "planning": args.plan,
                    "stage_kernels": names},
         "hbm_tbs_algorithmic": round(tbs, 3),
         "stage_ms": [round(t, 4) for t in stage_ms],

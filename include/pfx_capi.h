@@ -133,6 +133,11 @@ enum pfx_plan_flag { PFX_PLAN_ESTIMATE = 0, PFX_PLAN_MEASURE = 1, PFX_PLAN_PATIE
  */
 int pfx_plan_tune(pfx_plan plan, int flag, void *scratch, pfx_stream stream);
 
+/* Number of candidate kernels for `plan` (the set pfx_plan_tune times). */
+int pfx_plan_num_candidates(pfx_plan plan, int *count);
+/* Switches `plan` to candidate `index` (0 <= index < count). */
+int pfx_plan_use_candidate(pfx_plan plan, int index);
+
 /*
  * Saves / loads / clears the tuning cache (text, one entry per line).
  * Replaces: FFTW wisdom export / import (backend/fftw3/fft_backend_fftw.hpp:666-694).

@@ -58,6 +58,8 @@ SIGNATURES = [
     ("pfx_plan_destroy", _i, [_vp]),
     ("pfx_plan_describe", _i, [_vp, C.c_char_p, _sz, _ip, _ip]),
     ("pfx_plan_tune", _i, [_vp, _i, _vp, _vp]),
+    ("pfx_plan_num_candidates", _i, [_vp, _ip]),
+    ("pfx_plan_use_candidate", _i, [_vp, _i]),
     ("pfx_tune_cache_save", _i, [C.c_char_p]),
     ("pfx_tune_cache_load", _i, [C.c_char_p]),
     ("pfx_tune_cache_clear", _i, []),

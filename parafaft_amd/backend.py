@@ -120,6 +120,15 @@ class Plan:
         A.check(A.load().pfx_plan_tune(self.handle, int(flag), ptr(scratch), ptr(stream)),
                 "pfx_plan_tune")
 
+    def num_candidates(self) -> int:
+        n = C.c_int()
+        A.check(A.load().pfx_plan_num_candidates(self.handle, C.byref(n)),
+                "pfx_plan_num_candidates")
+        return n.value
+
+    def use_candidate(self, index: int):
+        A.check(A.load().pfx_plan_use_candidate(self.handle, index), "pfx_plan_use_candidate")
+
     def describe(self):
         L = A.load()
         buf = C.create_string_buffer(256)

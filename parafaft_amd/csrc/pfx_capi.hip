@@ -415,6 +415,21 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
   return PFX_SUCCESS;
 }
 
+int pfx_plan_num_candidates(pfx_plan p, int *count) {
+  if (!p || !count) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  *count = (int)candidates(p).size();
+  return PFX_SUCCESS;
+}
+
+int pfx_plan_use_candidate(pfx_plan p, int index) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  std::vector<const KernelEntry *> c = candidates(p);
+  if (index < 0 || index >= (int)c.size())
+    return fail(PFX_ERR_INVALID_ARG, "candidate %d out of range (%zu)", index, c.size());
+  PFX_HIP(set_kernel(p, c[index]), "candidate kernel");
+  return PFX_SUCCESS;
+}
+
 int pfx_tune_cache_save(const char *path) {
   if (!path) return fail(PFX_ERR_INVALID_ARG, "path is null");
   FILE *f = fopen(path, "w");
@@ -473,9 +488,9 @@ int pfx_plan_describe(pfx_plan p, char *name, size_t name_len, int *per_block, i
   if (p->kernel) {
     const KernelEntry *k = p->kernel;
     static const char *pn[] = {"f32", "f64", "f32split_f16", "f32split_bf16"};
-    snprintf(buf, sizeof(buf), "%s_%s_%s_n%d_r[%s]", pn[k->prec & 3],
+    snprintf(buf, sizeof(buf), "%s_%s_%s_n%d_r[%s]_c%d_t%d", pn[k->prec & 3],
              k->kind == KIND_C2C ? "c2c" : (k->kind == KIND_R2C ? "r2c" : "c2r"),
-             k->col ? "col" : "row", k->n, k->radices);
+             k->col ? "col" : "row", k->n, k->radices, k->per_block, k->tpt);
     if (per_block) *per_block = k->per_block;
     if (threads) *threads = k->tpt * k->per_block;
   } else {

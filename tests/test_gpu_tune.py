@@ -53,3 +53,39 @@ def test_backend_measure_flag_through_driver():
     fft.forward(d)
     assert rel_l2(to_host(d), O.fftn(x.astype(np.complex128))) <= tol("f32", n ** 3)
     B.tune_cache_clear()
+
+
+GEOMS = [  # (prec, n, batch, stride, dist, outer, outer_dist)
+    ("f32", 1024, 100, 1, 1024, 1, 0), ("f32", 1024, 75, 75, 1, 1, 0),
+    ("f32", 1024, 40, 40, 1, 3, 1024 * 40), ("f32", 2048, 50, 1, 2048, 1, 0),
+    ("f32", 2048, 33, 33, 1, 1, 0), ("f32", 4096, 20, 20, 1, 1, 0), ("f32", 512, 70, 70, 1, 1, 0),
+    ("f32", 256, 129, 129, 1, 1, 0), ("f64", 1024, 40, 40, 1, 1, 0), ("f64", 1024, 33, 1, 1024, 1, 0),
+    ("f32_split_f16", 1024, 50, 50, 1, 1, 0), ("f32_split_bf16", 1024, 50, 1, 1024, 1, 0),
+]
+
+
+@pytest.mark.parametrize("geom", GEOMS, ids=lambda g: "-".join(map(str, g[:5])))
+def test_every_candidate_kernel(geom):
+    """Every kernel the tuner may pick for a geometry gives the oracle's result."""
+    import parafaft_amd as P
+    prec, n, batch, stride, dist, outer, odist = geom
+    cdt = np.complex128 if prec == "f64" else np.complex64
+    size = (outer - 1) * odist + (batch - 1) * dist + (n - 1) * stride + 1
+    x = O.uniform_complex((size,), seed=n + batch).astype(cdt)
+    ref = x.astype(np.complex128).copy()
+    for o in range(outer):
+        seg = np.ascontiguousarray(ref[o * odist:])
+        O.c2c_batched(seg, n, batch, stride, dist, -1)
+        ref[o * odist:] = seg
+    plan = P.Plan.c2c(prec, n, batch, stride, dist, outer, odist)
+    k = plan.num_candidates()
+    assert k >= 1
+    names = set()
+    for i in range(k):
+        plan.use_candidate(i)
+        name = plan.describe()[0]
+        names.add(name)
+        d = to_dev(x)
+        plan.execute(0, d)
+        assert rel_l2(to_host(d), ref) <= tol("f64" if prec == "f64" else "f32", n), name
+    assert len(names) == k
