@@ -61,9 +61,12 @@ def cpu_baseline(n: int, frac_log2: int = 3):
 def bench_pencil(args, torch, A):
     """Times ParaFaFT_R2C<3>::forward_in_place for rank (0,0) of a 2048^3 real
     grid on a 4x4 process grid: R2C on the 512x512x2048 share, then per
-    transition pack -> self copy -> unpack -> C2C, peers' blocks resident."""
+    transition pack -> self copy -> unpack -> C2C (--unfused), or the fused
+    form where each stage stores into and loads from the exchange segments;
+    peers' blocks resident."""
     from parafaft_amd.pencil import PencilShareR2C
-    share = PencilShareR2C([2048] * 3, [4, 4], [0, 0], precision=args.precision)
+    share = PencilShareR2C([2048] * 3, [4, 4], [0, 0], precision=args.precision,
+                           fused=not args.unfused)
     rdt = torch.float64 if share.esz == 16 else torch.float32
     buf = torch.zeros(share.get_required_output_size(), dtype=rdt, device="cuda")
     g = torch.Generator(device="cuda")
@@ -82,7 +85,10 @@ def bench_pencil(args, torch, A):
                 torch.cuda.synchronize()
                 del blk
     torch.cuda.synchronize()
-    labels = ["r2c", "pack0", "unpack0", "fft1", "pack1", "unpack1", "fft2", "end"]
+    if share.fused:
+        labels = ["r2c", "fft1", "fft2", "end"]
+    else:
+        labels = ["r2c", "pack0", "unpack0", "fft1", "pack1", "unpack1", "fft2", "end"]
     stream = torch.cuda.current_stream()
     share.stream = stream.cuda_stream
     for _ in range(args.warmup):
@@ -110,10 +116,13 @@ def bench_pencil(args, torch, A):
     cplx2 = 512 * 2048 * 257 * esz
     alg = {"r2c": real_bytes + cplx, "pack0": 2 * cplx, "unpack0": 2 * cplx2, "fft1": 2 * cplx2,
            "pack1": 2 * cplx2, "unpack1": 2 * cplx2, "fft2": 2 * cplx2}
+    alg = {k: v for k, v in alg.items() if k in parts}
     total = sum(alg.values())
+    how = ("fused: stages store into / load from 4+4 device segments per transition"
+           if share.fused else "pack/unpack into 4+4 device segments per transition")
     print(json.dumps({
         "workload": "R2C pencil share: rank (0,0) of 2048^3 on a 4x4 grid, 512x512x2048 real, "
-                    "pack/unpack into 4+4 device segments per transition",
+                    + how, "fused": share.fused,
         "precision": args.precision, "steps": args.steps, "ms_per_forward": round(ms, 4),
         "part_ms": {k: round(v, 4) for k, v in parts.items()},
         "part_tbs": {k: round(alg[k] / (parts[k] * 1e-3) / 1e12, 3) for k in parts},
@@ -214,6 +223,8 @@ def main():
     ap.add_argument("--workload", default="cube", choices=["cube", "pencil", "batch"],
                     help="cube: the 1024^3 C2C headline; pencil: one rank's share of a "
                          "2048^3 R2C on a 4x4 grid with on-device pack/unpack")
+    ap.add_argument("--unfused", action="store_true",
+                    help="pencil workload: separate pack / self copy / unpack launches")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 4 GiB (the library's 2D-copy "
                          "kernel) five times, to scale rocprof FETCH/WRITE_SIZE counters")

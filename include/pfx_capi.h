@@ -111,6 +111,35 @@ int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream);
 int pfx_execute_scaled(pfx_plan plan, int direction, void *data, double scale,
                        pfx_stream stream);
 
+/*
+ * One axis of a row-major block split into per-peer segments, in the order
+ * the pencil exchange packs it (parafaft_common.hpp:576-590): element
+ * (lead, e, trail), e on the split axis, is complex element
+ * (lead * count[p] + e - start[p]) * trailing + trail of segments[p], p being
+ * the peer whose [start[p], start[p] + count[p]) holds e.  The blocks must
+ * tile the transform length in ascending order.  Arrays are host arrays of
+ * nparts (1..16) entries; segments[p] are device addresses.
+ */
+typedef struct {
+  int nparts;
+  void *const *segments;
+  const int *start;
+  const int *count;
+  uint64_t trailing;
+} pfx_segmap;
+
+/*
+ * pfx_execute with the pencil exchange fused into the transform: when `out`
+ * is given the last pass stores straight into the send segments (the pack of
+ * parafaft_common.hpp:576-590), when `in` is given the first pass loads from
+ * the receive segments (the unpack of :654-668).  Transform b of the plan is
+ * (lead, trail) = (b / trailing, b % trailing) of the map.  Results are
+ * bit-identical to execute-then-pack / unpack-then-execute.  Tiled C2C and
+ * R2C (`out` only) plans of plain precision.
+ */
+int pfx_execute_segmented(pfx_plan plan, int direction, void *data, const pfx_segmap *in,
+                          const pfx_segmap *out, double scale, pfx_stream stream);
+
 int pfx_plan_destroy(pfx_plan plan);
 
 /*

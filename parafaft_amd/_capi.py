@@ -55,6 +55,7 @@ SIGNATURES = [
     ("pfx_plan_create_c2r", _i, [_vpp, _i, _i64, _i64, _i64, _i64]),
     ("pfx_execute", _i, [_vp, _i, _vp, _vp]),
     ("pfx_execute_scaled", _i, [_vp, _i, _vp, _d, _vp]),
+    ("pfx_execute_segmented", _i, [_vp, _i, _vp, _vp, _vp, _d, _vp]),
     ("pfx_plan_destroy", _i, [_vp]),
     ("pfx_plan_describe", _i, [_vp, C.c_char_p, _sz, _ip, _ip]),
     ("pfx_plan_tune", _i, [_vp, _i, _vp, _vp]),
@@ -91,6 +92,13 @@ SIGNATURES = [
 ]
 
 
+class SegMap(C.Structure):
+    """pfx_segmap"""
+    _fields_ = [("nparts", C.c_int), ("segments", C.POINTER(C.c_void_p)),
+                ("start", C.POINTER(C.c_int)), ("count", C.POINTER(C.c_int)),
+                ("trailing", C.c_uint64)]
+
+
 class BatchDesc(C.Structure):
     """pfx_batch_desc"""
     _fields_ = [("kind", C.c_int), ("ndim", C.c_int), ("shape", C.c_int * 4),
@@ -117,6 +125,14 @@ def load():
             raise RuntimeError(
                 f"{LIB_PATH} is missing: the HIP library must be built "
                 "(python -m parafaft_amd.build); there is no CPU fallback")
+        # torch ships its own HIP runtime under the same soname as the system
+        # one libpfx links; whichever loads first serves the whole process, and
+        # torch cannot initialise on the other.  Load torch's first so device
+        # buffers from torch and libpfx's launches share one runtime.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, res, args in SIGNATURES:
             f = getattr(L, name)

@@ -26,6 +26,10 @@ CFLAGS = [
     # packed FP32 ops stay off in device code (hipcc's host pass warns it does
     # not know the feature; that warning is harmless)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fuse only inside one source expression, so every kernel
+    # instance of the same butterflies rounds identically (plain, segmented
+    # and grouped forms give bit-identical results)
+    "-ffp-contract=on",
     "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
     f"-I{os.path.join(ROOT, 'include')}", f"-I{CSRC}",
 ]

@@ -61,9 +61,11 @@ const Registry &registry() {
 // XCD-aware tile order for batch-contiguous kernels (PFX_XCD_REMAP=0 turns it
 // off, for measurement).
 // Both are read when a plan is created.
-bool xcd_remap_enabled() {
+// PFX_XCD_REMAP=0: launch order; 1 (default): one contiguous run of tiles
+// per XCD; K > 1: XCDs take turns on chunks of K tiles.
+int xcd_remap_mode() {
   const char *s = getenv("PFX_XCD_REMAP");
-  return !(s && s[0] == '0');
+  return s ? atoi(s) : 1;
 }
 
 // Override of the batch-contiguous tile: PFX_COL_TILE=<transforms per workgroup>.
@@ -126,6 +128,15 @@ const KernelEntry *entry_by_id(const std::string &id) {
   for (const KernelEntry *e : registry().entries)
     if (entry_id(e) == id) return e;
   return nullptr;
+}
+
+// A cache value is "<entry id>|order=<tile order>".
+bool parse_tuned(const std::string &v, const KernelEntry **k, int *order) {
+  const size_t pos = v.rfind("|order=");
+  const std::string id = pos == std::string::npos ? v : v.substr(0, pos);
+  *order = pos == std::string::npos ? 1 : atoi(v.c_str() + pos + 7);
+  *k = entry_by_id(id);
+  return *k != nullptr;
 }
 
 std::string plan_key(const pfx_plan_t *p) {
@@ -194,7 +205,7 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   p->stride = stride;
   p->dist = dist;
   p->dist_outer = dist_outer;
-  p->xcd_remap = xcd_remap_enabled() ? 1 : 0;
+  p->xcd_remap = xcd_remap_mode();
 
   const int bp = base_prec(precision);
   const size_t esz = bp ? 16 : 8;
@@ -214,8 +225,12 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   if (p->kernel) {  // a tuned choice for this geometry wins over the table
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find(plan_key(p));
-    if (it != g_tune.end())
-      if (const KernelEntry *e = entry_by_id(it->second)) p->kernel = e;
+    const KernelEntry *e = nullptr;
+    int order = 1;
+    if (it != g_tune.end() && parse_tuned(it->second, &e, &order)) {
+      p->kernel = e;
+      p->xcd_remap = order;
+    }
   }
   if (!p->kernel) {
     if (length > naive_max_length(bp)) {
@@ -340,6 +355,7 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
     a.use_scale = scale != 1.0;
     a.backward = direction == PFX_BACKWARD;
     a.xcd_remap = p->xcd_remap;
+    a.in_seg = a.out_seg = 0;
     p->kernel->launch(a, s);
   } else {
     NaiveArgs a;
@@ -362,6 +378,79 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
   return PFX_SUCCESS;
 }
 
+static int to_segmap(const pfx_segmap *m, int length, int64_t total, SegMap *out) {
+  if (m->nparts < 1 || m->nparts > kMaxSegPeers || !m->segments || !m->start || !m->count ||
+      m->trailing == 0)
+    return fail(PFX_ERR_INVALID_ARG, "bad segment map (nparts must be 1..%d)", kMaxSegPeers);
+  out->nparts = m->nparts;
+  int end = 0;
+  for (int q = 0; q < m->nparts; ++q) {
+    if (m->start[q] != end || m->count[q] < 0)
+      return fail(PFX_ERR_INVALID_ARG, "segment blocks must tile the axis in order");
+    if (m->count[q] > 0 && !m->segments[q])
+      return fail(PFX_ERR_INVALID_ARG, "segment %d has no address", q);
+    end += m->count[q];
+    out->s[q] = m->start[q];
+    out->n[q] = m->count[q];
+    out->seg[q] = m->segments[q];
+  }
+  if (end != length)
+    return fail(PFX_ERR_INVALID_ARG, "segment blocks cover %d of %d elements", end, length);
+  // the kernels index a segment's rows (lead * count + e) in 32 bits
+  int maxn = 0;
+  for (int q = 0; q < m->nparts; ++q) maxn = std::max(maxn, m->count[q]);
+  const int64_t leads = (total + (int64_t)m->trailing - 1) / (int64_t)m->trailing;
+  if (m->trailing >= (1ull << 32) || (leads + 1) * (int64_t)maxn >= (1ll << 31))
+    return fail(PFX_ERR_UNSUPPORTED, "segment of %lld rows exceeds 32-bit row indexing",
+                (long long)((leads + 1) * maxn));
+  out->trailing = (int64_t)m->trailing;
+  return PFX_SUCCESS;
+}
+
+int pfx_execute_segmented(pfx_plan p, int direction, void *data, const pfx_segmap *in,
+                          const pfx_segmap *out, double scale, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (!in && !out) return pfx_execute_scaled(p, direction, data, scale, stream);
+  if (!p->kernel || p->kernel->prec > 1)
+    return fail(PFX_ERR_UNSUPPORTED, "segmented execution needs a tiled plain-precision kernel");
+  if (p->kind == PFX_C2R || (p->kind == PFX_R2C && in))
+    return fail(PFX_ERR_UNSUPPORTED, "segmented loads are for C2C plans; C2R is not fused");
+  if ((p->kind == PFX_R2C && direction != PFX_FORWARD) ||
+      (direction != PFX_FORWARD && direction != PFX_BACKWARD))
+    return fail(PFX_ERR_INVALID_ARG, "bad direction");
+  const int64_t total = p->inner * p->outer;
+  if (total == 0) return PFX_SUCCESS;
+  if ((!in || !out) && !data) return fail(PFX_ERR_INVALID_ARG, "data is null");
+  const bool real = p->kind != PFX_C2C;
+  StockhamArgs a;
+  a.data = data;
+  a.tw = p->tw;
+  a.ptw = p->ptw;
+  a.total = total;
+  a.inner = p->inner;
+  a.stride = p->stride;
+  a.dist = real ? p->dist / 2 : p->dist;
+  a.dist_outer = real ? p->dist_outer / 2 : p->dist_outer;
+  a.scale = scale;
+  a.use_scale = scale != 1.0;
+  a.backward = direction == PFX_BACKWARD;
+  a.xcd_remap = p->xcd_remap;
+  a.in_seg = in ? 1 : 0;
+  a.out_seg = out ? 1 : 0;
+  if (in) {
+    int rc = to_segmap(in, (int)p->length, total, &a.in_map);
+    if (rc) return rc;
+  }
+  if (out) {
+    int rc = to_segmap(out, real ? (int)p->length / 2 + 1 : (int)p->length, total, &a.out_map);
+    if (rc) return rc;
+  }
+  p->kernel->launch(a, static_cast<hipStream_t>(stream));
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "kernel launch");
+  return PFX_SUCCESS;
+}
+
 int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream) {
   return pfx_execute_scaled(plan, direction, data, 1.0, stream);
 }
@@ -376,11 +465,12 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune.find(key);
-    if (it != g_tune.end()) {
-      if (const KernelEntry *e = entry_by_id(it->second)) {
-        PFX_HIP(set_kernel(p, e), "tuned kernel");
-        return PFX_SUCCESS;
-      }
+    const KernelEntry *e = nullptr;
+    int order = 1;
+    if (it != g_tune.end() && parse_tuned(it->second, &e, &order)) {
+      PFX_HIP(set_kernel(p, e), "tuned kernel");
+      p->xcd_remap = order;
+      return PFX_SUCCESS;
     }
   }
   const int reps = flag == PFX_PLAN_PATIENT ? 10 : 3;
@@ -391,27 +481,37 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
   const KernelEntry *best = p->kernel;
   float best_ms = -1.f;
   const int dir = p->kind == PFX_C2R ? PFX_BACKWARD : PFX_FORWARD;
+  int best_order = p->xcd_remap;
   for (const KernelEntry *k : candidates(p)) {
     hipError_t e = set_kernel(p, k);
     if (e != hipSuccess) continue;
-    int rc = pfx_execute(p, dir, scratch, stream);  // warm-up
-    if (rc != PFX_SUCCESS) continue;
-    hipEventRecord(a, s);
-    for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r) rc = pfx_execute(p, dir, scratch, stream);
-    hipEventRecord(b, s);
-    if (hipEventSynchronize(b) != hipSuccess || rc != PFX_SUCCESS) continue;
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, a, b);
-    if (best_ms < 0.f || ms < best_ms) {
-      best_ms = ms;
-      best = k;
+    // batch-contiguous kernels also try XCD tile orders (contiguous runs per
+    // XCD, or XCDs taking turns on chunks of 8 / 32 tiles)
+    std::vector<int> orders = {p->xcd_remap};
+    if (k->col) orders = {1, 8, 32};
+    for (int order : orders) {
+      p->xcd_remap = order;
+      int rc = pfx_execute(p, dir, scratch, stream);  // warm-up
+      if (rc != PFX_SUCCESS) continue;
+      hipEventRecord(a, s);
+      for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r) rc = pfx_execute(p, dir, scratch, stream);
+      hipEventRecord(b, s);
+      if (hipEventSynchronize(b) != hipSuccess || rc != PFX_SUCCESS) continue;
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, a, b);
+      if (best_ms < 0.f || ms < best_ms) {
+        best_ms = ms;
+        best = k;
+        best_order = order;
+      }
     }
   }
   hipEventDestroy(a);
   hipEventDestroy(b);
   PFX_HIP(set_kernel(p, best), "tuned kernel");
+  p->xcd_remap = best_order;
   std::lock_guard<std::mutex> lock(g_tune_mu);
-  g_tune[key] = entry_id(best);
+  g_tune[key] = entry_id(best) + "|order=" + std::to_string(best_order);
   return PFX_SUCCESS;
 }
 
@@ -455,7 +555,9 @@ int pfx_tune_cache_load(const char *path) {
     *tab = 0;
     std::string val(tab + 1);
     while (!val.empty() && (val.back() == '\n' || val.back() == '\r')) val.pop_back();
-    if (!entry_by_id(val)) { ++bad; continue; }  // kernel no longer in this build
+    const KernelEntry *ke = nullptr;
+    int order = 1;
+    if (!parse_tuned(val, &ke, &order)) { ++bad; continue; }  // kernel not in this build
     g_tune[line] = val;
   }
   fclose(f);
@@ -762,6 +864,7 @@ StockhamArgs job_args(pfx_plan p, void *data, int direction) {
   a.use_scale = 0;
   a.backward = direction == PFX_BACKWARD;
   a.xcd_remap = p->xcd_remap;
+  a.in_seg = a.out_seg = 0;
   return a;
 }
 

@@ -36,12 +36,18 @@ template <int... Rs> constexpr int prod() { return (Rs * ... * 1); }
 template <typename T, int TPT, int C, bool COL, int... Rs>
 void launch_c2c(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
-  c2c_kernel<T, TPT, C, COL, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  if (a.in_seg || a.out_seg)
+    c2c_kernel<T, TPT, C, COL, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  else
+    c2c_kernel<T, TPT, C, COL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, int... Rs>
 void launch_c2c_wide(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
-  c2c_wide_kernel<T, TPT, C, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  if (a.in_seg || a.out_seg)
+    c2c_wide_kernel<T, TPT, C, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  else
+    c2c_wide_kernel<T, TPT, C, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, bool COL, int... Rs>
 void group_c2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
@@ -62,7 +68,10 @@ void group_c2r(const GroupArgs &g, int64_t blocks, hipStream_t s) {
 template <typename T, int TPT, int C, int... Rs>
 void launch_r2c(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
-  r2c_kernel<T, TPT, C, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  if (a.out_seg)
+    r2c_kernel<T, TPT, C, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  else
+    r2c_kernel<T, TPT, C, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, int... Rs>
 void launch_c2r(const StockhamArgs &a, hipStream_t s) {

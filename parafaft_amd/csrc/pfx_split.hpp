@@ -260,7 +260,7 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
   int64_t tile = blockIdx.x;
-  if (COL && a.xcd_remap) tile = xcd_tile(tile, gridDim.x);
+  if (COL) tile = tile_order(tile, gridDim.x, a.xcd_remap);
   const int64_t b = tile * C + c;
   const bool valid = b < a.total;
   V2<float>* p = static_cast<V2<float>*>(a.data) + (valid ? xform_offset(b, a) : 0);

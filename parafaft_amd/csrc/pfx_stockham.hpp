@@ -26,6 +26,7 @@
 // An optional scale is fused into the final HBM store.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <stddef.h>
 #include <stdint.h>
 #include "pfx_butterfly.hpp"
 
@@ -34,6 +35,47 @@ namespace pfx {
 enum Kind { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
 
 template <int R, int... Rest> constexpr int first_of() { return R; }
+
+// One axis of a row-major block split into per-peer segments, in the order
+// the pencil exchange packs it (parafaft_common.hpp:576-590): element
+// (lead, e, trail) of the block, with e on the split axis, lives at
+//   seg[p][(lead * n[p] + (e - s[p])) * trailing + trail]
+// p being the peer whose block [s[p], s[p] + n[p]) holds e.  Every segment
+// has its own address, so the rank's own block can sit in its receive
+// segment while the others sit in the send buffer.  A stage's last pass can
+// store straight into this layout (the pack) and the next stage's first
+// pass can load from it (the unpack).
+constexpr int kMaxSegPeers = 16;
+struct SegMap {
+  int nparts;
+  int s[kMaxSegPeers];
+  int n[kMaxSegPeers];
+  void* seg[kMaxSegPeers];
+  int64_t trailing;
+};
+
+// Address of element e of the transform at (lead, trail); a transform's
+// (lead, trail) is split from its index once, at set-up.  The peer is found
+// by a uniform scan with selects (no per-lane indexing into the map, which
+// would move the kernel arguments to scratch); the host guarantees
+// lead * n[p] + n[p] < 2^31 and trailing < 2^32.
+// Maps are read in place from the kernel-argument segment (constant address
+// space, scalar loads).
+using KSegMap = const __attribute__((address_space(4))) SegMap;
+
+template <typename T>
+__device__ __forceinline__ V2<T>* seg_addr(KSegMap* m, uint32_t lead, int64_t trail, int e) {
+  V2<T>* base = static_cast<V2<T>*>(m->seg[0]);
+  int nn = m->n[0], ss = 0;
+  for (int q = 1; q < m->nparts; ++q) {
+    const bool here = e >= m->s[q];
+    base = here ? static_cast<V2<T>*>(m->seg[q]) : base;
+    nn = here ? m->n[q] : nn;
+    ss = here ? m->s[q] : ss;
+  }
+  const uint32_t row = lead * (uint32_t)nn + (uint32_t)(e - ss);
+  return base + ((uint64_t)row * (uint64_t)m->trailing + trail);
+}
 
 struct StockhamArgs {
   void* data;            // complex view of the buffer (R2C/C2R: padded rows)
@@ -48,7 +90,11 @@ struct StockhamArgs {
   double scale;          // multiplies every output (1.0 = none)
   int backward;          // C2C only: 1 = exp(+i...), via re/im swap
   int use_scale;
-  int xcd_remap;         // 1: give each XCD a contiguous run of tiles
+  int xcd_remap;         // tile order: 0 launch order; 1 one contiguous run of
+                         // tiles per XCD; K > 1: XCDs take turns on chunks of K
+  int in_seg;            // 1: first pass loads from in_map (fused unpack)
+  int out_seg;           // 1: last pass stores into out_map (fused pack)
+  SegMap in_map, out_map;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs (workgroup i runs on
@@ -60,6 +106,19 @@ __device__ inline int64_t xcd_tile(int64_t i, int64_t nb) {
   const int64_t q = nb / kNumXcd, r = nb % kNumXcd;
   const int64_t x = i % kNumXcd, k = i / kNumXcd;
   return x * q + (x < r ? x : r) + k;
+}
+// XCD x takes chunks x, x + 8, x + 16, ... of K consecutive tiles, so all
+// XCDs work inside one window of 8 K tiles at a time.  Used when 8 K divides
+// the tile count; otherwise the contiguous-run order.
+__device__ inline int64_t xcd_tile_chunked(int64_t i, int64_t nb, int64_t K) {
+  if (K <= 1 || nb % (kNumXcd * K) != 0) return xcd_tile(i, nb);
+  const int64_t x = i % kNumXcd, k = i / kNumXcd;
+  return ((k / K) * kNumXcd + x) * K + (k % K);
+}
+__device__ inline int64_t tile_order(int64_t i, int64_t nb, int mode) {
+  if (mode == 0) return i;
+  if (mode == 1) return xcd_tile(i, nb);
+  return xcd_tile_chunked(i, nb, mode);
 }
 
 template <typename T, int N, int C, bool COL> struct LdsLayout {
@@ -115,14 +174,75 @@ __device__ inline void stockham_pass(int j, const V2<T>* __restrict__ stw, LoadF
   store.after();
 }
 
-template <typename T> struct GlobalPtr {
+// HBM side of one transform.  SEG = false is the plain strided row/column;
+// SEG = true adds the segment maps of the fused exchange (a separate kernel
+// instance, so the plain kernels carry none of its addressing).
+template <typename T, bool SEG = false> struct GlobalPtr {
   V2<T>* p;
   int64_t stride;
   bool swap_io;
   bool valid;
   T scale;
   bool use_scale;
+  __device__ inline V2<T> load(int e) const { return p[(int64_t)e * stride]; }
+  __device__ inline void store(int e, V2<T> v) const { p[(int64_t)e * stride] = v; }
 };
+
+template <typename T> struct GlobalPtr<T, true> {
+  V2<T>* p;
+  int64_t stride;
+  bool swap_io;
+  bool valid;
+  T scale;
+  bool use_scale;
+  KSegMap* in_map;           // non-null: loads come from a segment layout
+  KSegMap* out_map;          // non-null: stores go to a segment layout
+  uint32_t in_lead, out_lead;  // transform's place in each map
+  int64_t in_trail, out_trail;
+  __device__ inline V2<T> load(int e) const {
+    if (in_map) return *seg_addr<T>(in_map, in_lead, in_trail, e);
+    return p[(int64_t)e * stride];
+  }
+  __device__ inline void store(int e, V2<T> v) const {
+    if (out_map) *seg_addr<T>(out_map, out_lead, out_trail, e) = v;
+    else p[(int64_t)e * stride] = v;
+  }
+};
+
+__device__ inline int64_t xform_offset(int64_t b, const StockhamArgs& a) {
+  const int64_t o = b / a.inner;
+  const int64_t i = b - o * a.inner;
+  return o * a.dist_outer + i * a.dist;
+}
+
+template <typename T, bool SEG>
+__device__ inline void init_global(GlobalPtr<T, SEG>& g, const StockhamArgs& a, int64_t b) {
+  g.valid = b < a.total;
+  if constexpr (!SEG) {
+    g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  } else {
+    g.p = static_cast<V2<T>*>(a.data) +
+          (g.valid && !(a.in_seg && a.out_seg) ? xform_offset(b, a) : 0);
+    // Only single-plan kernels run segmented, and their one argument (the
+    // StockhamArgs) sits at offset 0 of the kernel-argument segment; taking
+    // the address of the parameter itself would copy it to scratch.
+    const auto* kern =
+        (const __attribute__((address_space(4))) char*)__builtin_amdgcn_kernarg_segment_ptr();
+    g.in_map = a.in_seg ? reinterpret_cast<KSegMap*>(kern + offsetof(StockhamArgs, in_map)) : nullptr;
+    g.out_map = a.out_seg ? reinterpret_cast<KSegMap*>(kern + offsetof(StockhamArgs, out_map)) : nullptr;
+    g.in_lead = g.in_trail = g.out_lead = g.out_trail = 0;
+    if (a.in_seg) {
+      const int64_t l = b / a.in_map.trailing;
+      g.in_lead = (uint32_t)l;
+      g.in_trail = b - l * a.in_map.trailing;
+    }
+    if (a.out_seg) {
+      const int64_t l = b / a.out_map.trailing;
+      g.out_lead = (uint32_t)l;
+      g.out_trail = b - l * a.out_map.trailing;
+    }
+  }
+}
 
 // Run passes PI.. of the radix list.  The first pass may read HBM (FIRST_G),
 // the last may write HBM (LAST_G); everything else goes through LDS.
@@ -133,20 +253,21 @@ struct Passes;
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
           int Ns>
 struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns> {
-  __device__ static inline void run(int, int, V2<T>*, const V2<T>*, const GlobalPtr<T>&) {}
+  template <class G> __device__ static inline void run(int, int, V2<T>*, const V2<T>*, const G&) {}
 };
 
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
           int Ns, int R, int... Rest>
 struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
   using L = LdsLayout<T, N, C, COL>;
-  __device__ static inline void run(int j, int c, V2<T>* lds, const V2<T>* tw, const GlobalPtr<T>& g) {
+  template <class G>
+  __device__ static inline void run(int j, int c, V2<T>* lds, const V2<T>* tw, const G& g) {
     constexpr bool IS_FIRST = (PI == 0);
     constexpr bool IS_LAST = (sizeof...(Rest) == 0);
     auto load = [&](int e) -> V2<T> {
       if constexpr (IS_FIRST && FIRST_G) {
         V2<T> x = mk<T>(T(0), T(0));
-        if (g.valid) x = g.p[e * g.stride];
+        if (g.valid) x = g.load(e);
         return g.swap_io ? swp<T>(x) : x;
       } else {
         return lds[L::at(e, c)];
@@ -159,7 +280,7 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
       __device__ void operator()(int e, V2<T> v) const { lds[L::at(e, c)] = v; }
     };
     struct StoreG {
-      const GlobalPtr<T>* g;
+      const G* g;
       // A single pass that both reads and writes HBM in place must see every
       // lane's loads done before any lane stores (only matters when TPT > 1).
       __device__ void before() const {
@@ -169,7 +290,7 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
       __device__ void operator()(int e, V2<T> v) const {
         if (g->use_scale) v = scl<T>(v, g->scale);
         if (g->swap_io) v = swp<T>(v);
-        if (g->valid) g->p[e * g->stride] = v;
+        if (g->valid) g->store(e, v);
       }
     };
     if constexpr (IS_LAST && LAST_G) {
@@ -200,19 +321,13 @@ __device__ inline void thread_coords(int& j, int& c) {
   else { j = tid % TPT; c = tid / TPT; }
 }
 
-__device__ inline int64_t xform_offset(int64_t b, const StockhamArgs& a) {
-  const int64_t o = b / a.inner;
-  const int64_t i = b - o * a.inner;
-  return o * a.dist_outer + i * a.dist;
-}
-
 // ---------------------------------------------------------------------------
 // C2C
 // ---------------------------------------------------------------------------
 // Kernel bodies take the workgroup's tile id and the tile count of its
 // launch (or of its job in a grouped launch), so the single-plan kernels and
 // the grouped multi-transform kernels share them.
-template <typename T, int TPT, int C, bool COL, int... Rs>
+template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
 __device__ inline void c2c_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, COL>;
@@ -221,11 +336,10 @@ __device__ inline void c2c_body(const StockhamArgs& a, int64_t tile, int64_t nti
   const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
-  if (COL && a.xcd_remap) tile = xcd_tile(tile, ntiles);
+  if (COL) tile = tile_order(tile, ntiles, a.xcd_remap);
   const int64_t b = tile * C + c;
-  GlobalPtr<T> g;
-  g.valid = b < a.total;
-  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  GlobalPtr<T, SEG> g;
+  init_global(g, a, b);
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
@@ -233,9 +347,9 @@ __device__ inline void c2c_body(const StockhamArgs& a, int64_t tile, int64_t nti
   Passes<T, N, TPT, C, COL, 1, true, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
 }
 
-template <typename T, int TPT, int C, bool COL, int... Rs>
+template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_kernel(StockhamArgs a) {
-  c2c_body<T, TPT, C, COL, Rs...>(a, blockIdx.x, gridDim.x);
+  c2c_body<T, TPT, C, COL, SEG, Rs...>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -244,7 +358,7 @@ __global__ void __launch_bounds__(TPT * C) c2c_kernel(StockhamArgs a) {
 // k = 0..N, written as N+1 complex values over the same padded row.
 // Row layout only (stride 1).  a.dist is in complex units (= padded dist / 2).
 // ---------------------------------------------------------------------------
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool SEG, int... Rs>
 __device__ inline void r2c_body(const StockhamArgs& a, int64_t tile) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
@@ -254,9 +368,8 @@ __device__ inline void r2c_body(const StockhamArgs& a, int64_t tile) {
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = tile * C + c;
-  GlobalPtr<T> g;
-  g.valid = b < a.total;
-  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  GlobalPtr<T, SEG> g;
+  init_global(g, a, b);
   g.stride = 1;
   g.swap_io = false;
   g.scale = T(a.scale);
@@ -276,7 +389,7 @@ __device__ inline void r2c_body(const StockhamArgs& a, int64_t tile) {
       V2<T> wd = cmul<T>(tw[k], d);
       V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
       if (g.use_scale) x = scl<T>(x, g.scale);
-      if (g.valid) g.p[k] = x;
+      if (g.valid) g.store(k, x);
     }
     // X[N-k] unless k is its own mirror (2k == N); k = 0 gives X[N]
     if (2 * k != N) {
@@ -286,14 +399,14 @@ __device__ inline void r2c_body(const StockhamArgs& a, int64_t tile) {
       V2<T> wd = cmul<T>(tw[kk], d);
       V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
       if (g.use_scale) x = scl<T>(x, g.scale);
-      if (g.valid) g.p[kk] = x;
+      if (g.valid) g.store(kk, x);
     }
   }
 }
 
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) r2c_kernel(StockhamArgs a) {
-  r2c_body<T, TPT, C, Rs...>(a, blockIdx.x);
+  r2c_body<T, TPT, C, SEG, Rs...>(a, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -301,7 +414,7 @@ __global__ void __launch_bounds__(TPT * C) r2c_kernel(StockhamArgs a) {
 // Im X[0] and Im X[N] ignored (Hermitian input); z = IFFT_N(Z) (unnormalised)
 // gives z[n] = x[2n] + i x[2n+1], stored as the N complex slots of the row.
 // ---------------------------------------------------------------------------
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool SEG, int... Rs>
 __device__ inline void c2r_body(const StockhamArgs& a, int64_t tile) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
@@ -311,9 +424,8 @@ __device__ inline void c2r_body(const StockhamArgs& a, int64_t tile) {
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = tile * C + c;
-  GlobalPtr<T> g;
-  g.valid = b < a.total;
-  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  GlobalPtr<T, SEG> g;
+  init_global(g, a, b);
   g.stride = 1;
   g.swap_io = true;  // inverse FFT via re/im swap; pre-twiddled data is stored swapped
   g.scale = T(a.scale);
@@ -322,7 +434,7 @@ __device__ inline void c2r_body(const StockhamArgs& a, int64_t tile) {
   for (int k = j; k <= N / 2; k += TPT) {
     const int km = N - k;
     V2<T> Xk = mk<T>(T(0), T(0)), Xm = Xk;
-    if (g.valid) { Xk = g.p[k]; Xm = g.p[km]; }
+    if (g.valid) { Xk = g.load(k); Xm = g.load(km); }
     if (k == 0) { Xk.y = T(0); Xm.y = T(0); }
     // Z[k]
     {
@@ -346,7 +458,7 @@ __device__ inline void c2r_body(const StockhamArgs& a, int64_t tile) {
 
 template <typename T, int TPT, int C, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2r_kernel(StockhamArgs a) {
-  c2r_body<T, TPT, C, Rs...>(a, blockIdx.x);
+  c2r_body<T, TPT, C, false, Rs...>(a, blockIdx.x);
 }
 
 }  // namespace pfx
@@ -366,14 +478,14 @@ struct WidePasses;
 
 template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns>
 struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns> {
-  __device__ static inline void run(int, int, V2<T>*, T*, const V2<T>*, const GlobalPtr<T>&) {}
+  template <class G> __device__ static __forceinline__ void run(int, int, V2<T>*, T*, const V2<T>*, const G&) {}
 };
 
 template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns, int R, int... Rest>
 struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
-  __device__ static inline void run(int j, int c, V2<T>* v, T* lds, const V2<T>* tw,
-                                    const GlobalPtr<T>& g) {
+  template <class G>
+  __device__ static __forceinline__ void run(int j, int c, V2<T>* v, T* lds, const V2<T>* tw, const G& g) {
     constexpr int Q = E / R;
     // twiddle + butterflies on the values this thread holds
 #pragma unroll
@@ -397,14 +509,16 @@ struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
           V2<T> x = v[q * R + m];
           if (g.use_scale) x = scl<T>(x, g.scale);
           if (g.swap_io) x = swp<T>(x);
-          if (g.valid) g.p[(int64_t)(ob + m * Ns) * g.stride] = x;
+          if (g.valid) g.store(ob + m * Ns, x);
         }
       }
     } else {
       constexpr int R2 = first_of<Rest...>();
       constexpr int Q2 = E / R2;
       constexpr int NB2 = N / R2;
-      V2<T> w[E];
+      // One component at a time: every lane writes its v[].x (or .y) to LDS,
+      // and after the barrier that component of v is dead, so the next
+      // pass's values are read straight back into it (no second array).
 #pragma unroll
       for (int comp = 0; comp < 2; ++comp) {
 #pragma unroll
@@ -422,21 +536,19 @@ struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
 #pragma unroll
           for (int m = 0; m < R2; ++m) {
             const T s = lds[(jj + m * NB2) * C + c];
-            if (comp) w[q * R2 + m].y = s;
-            else w[q * R2 + m].x = s;
+            if (comp) v[q * R2 + m].y = s;
+            else v[q * R2 + m].x = s;
           }
         }
         __syncthreads();
       }
-#pragma unroll
-      for (int i = 0; i < E; ++i) v[i] = w[i];
       WidePasses<T, N, TPT, C, E, TWS, PI + 1, Ns * R, Rest...>::run(j, c, v, lds, tw, g);
     }
   }
 };
 
-template <typename T, int TPT, int C, int... Rs>
-__device__ inline void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+template <typename T, int TPT, int C, bool SEG, int... Rs>
+__device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   constexpr int E = N / TPT;
   constexpr int R0 = first_of<Rs...>();
@@ -446,11 +558,10 @@ __device__ inline void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_
   __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
   const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   const int c = threadIdx.x % C, j = threadIdx.x / C;
-  if (a.xcd_remap) tile = xcd_tile(tile, ntiles);
+  tile = tile_order(tile, ntiles, a.xcd_remap);
   const int64_t b = tile * C + c;
-  GlobalPtr<T> g;
-  g.valid = b < a.total;
-  g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+  GlobalPtr<T, SEG> g;
+  init_global(g, a, b);
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
@@ -462,16 +573,16 @@ __device__ inline void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_
 #pragma unroll
     for (int m = 0; m < R0; ++m) {
       V2<T> x = mk<T>(T(0), T(0));
-      if (g.valid) x = g.p[(int64_t)(jj + m * NB0) * g.stride];
+      if (g.valid) x = g.load(jj + m * NB0);
       v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
     }
   }
   WidePasses<T, N, TPT, C, E, 1, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
 }
 
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_wide_kernel(StockhamArgs a) {
-  c2c_wide_body<T, TPT, C, Rs...>(a, blockIdx.x, gridDim.x);
+  c2c_wide_body<T, TPT, C, SEG, Rs...>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -500,23 +611,23 @@ template <typename T, int TPT, int C, bool COL, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_group_kernel(GroupArgs ga) {
   const int jb = find_job(ga, blockIdx.x);
   const int64_t s0 = ga.start[jb];
-  c2c_body<T, TPT, C, COL, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
+  c2c_body<T, TPT, C, COL, false, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
 }
 template <typename T, int TPT, int C, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_wide_group_kernel(GroupArgs ga) {
   const int jb = find_job(ga, blockIdx.x);
   const int64_t s0 = ga.start[jb];
-  c2c_wide_body<T, TPT, C, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
+  c2c_wide_body<T, TPT, C, false, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
 }
 template <typename T, int TPT, int C, int... Rs>
 __global__ void __launch_bounds__(TPT * C) r2c_group_kernel(GroupArgs ga) {
   const int jb = find_job(ga, blockIdx.x);
-  r2c_body<T, TPT, C, Rs...>(ga.jobs[jb], blockIdx.x - ga.start[jb]);
+  r2c_body<T, TPT, C, false, Rs...>(ga.jobs[jb], blockIdx.x - ga.start[jb]);
 }
 template <typename T, int TPT, int C, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2r_group_kernel(GroupArgs ga) {
   const int jb = find_job(ga, blockIdx.x);
-  c2r_body<T, TPT, C, Rs...>(ga.jobs[jb], blockIdx.x - ga.start[jb]);
+  c2r_body<T, TPT, C, false, Rs...>(ga.jobs[jb], blockIdx.x - ga.start[jb]);
 }
 
 }  // namespace pfx

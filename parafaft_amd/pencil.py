@@ -17,6 +17,14 @@ Every transition has nparts send and nparts receive segments, all
 device-resident: for a 3D share on a 4 x 4 grid that is 16 peer buffers.
 The peers' incoming blocks are whatever the caller places in the receive
 segments (`recv_segment`); nothing crosses to another GPU.
+
+Fused form (`fused=True`, the default where it applies): the pack and the
+unpack move into the transforms (pfx_execute_segmented).  Stage t's last
+pass stores its output straight into transition t's segments, the rank's
+own block straight into its receive segment (so no self copy), and stage
+t + 1's first pass loads from transition t's receive segments.  The
+segments hold bit-identical data to the unfused path; only the HBM round
+trips of pack, self copy and unpack are gone.
 """
 from __future__ import annotations
 
@@ -28,8 +36,14 @@ from .backend import DeviceBuffer, Plan, precision_code, ptr
 from .geometry import exchange_geometry, r2c_stage_shapes
 
 
+def _fusable(plan):
+    name = plan.describe()[0]
+    return "direct_dft" not in name and "split" not in name
+
+
 class PencilShareR2C:
-    def __init__(self, global_real_shape, dims, coords, precision="f32", fast=True, stream=None):
+    def __init__(self, global_real_shape, dims, coords, precision="f32", fast=True, stream=None,
+                 fused=None):
         self.D = D = len(global_real_shape)
         self.global_real = [int(v) for v in global_real_shape]
         self.dims = list(dims)
@@ -54,7 +68,6 @@ class PencilShareR2C:
                                                 D - 2 - t))
         self.send = [DeviceBuffer(max(1, sum(g.send_counts)) * self.esz) for g in self.geoms]
         self.recv = [DeviceBuffer(max(1, sum(g.recv_counts)) * self.esz) for g in self.geoms]
-        self.scratch = DeviceBuffer(self.max_complex * self.esz)
         # plans: stage 0 R2C, stages 1.. C2C (create_backend_plans, r2c:1158-1206)
         n = self.global_real[-1]
         batch0 = math.prod(ins[0][:D - 1])
@@ -82,6 +95,16 @@ class PencilShareR2C:
                     self.plans.append(Plan.c2c(self.precision, length, trailing, trailing, 1))
                     self.leading.append(leading)
                 self.slice.append(length * trailing * self.esz)
+        # the fused form needs one launch per stage (fast plans), tiled
+        # plain-precision kernels, and segment maps the kernels take (at most
+        # 16 peers per grid dimension)
+        tiled = all(_fusable(pl) for pl in [self.r2c] + self.plans[1:])
+        can_fuse = fast and tiled and all(1 <= n <= 16 for n in self.nparts)
+        self.fused = can_fuse if fused is None else bool(fused)
+        if self.fused and not can_fuse:
+            raise ValueError("the fused exchange needs fast=True, tiled FP32/FP64 kernels and "
+                             "at most 16 parts per grid dimension")
+        self.scratch = None if self.fused else DeviceBuffer(self.max_complex * self.esz)
 
     # sizes ---------------------------------------------------------------
     def get_required_output_size(self):
@@ -127,14 +150,57 @@ class PencilShareR2C:
                                     g.dst_trailing, g.dst_axis_extent, self.esz, self.stream),
                 "pfx_unpack")
 
+    def segmap(self, t, side):
+        """pfx_segmap of transition t: side "out" is where stage t stores
+        (send segments, the rank's own block in its receive segment), side
+        "in" where stage t + 1 loads (receive segments)."""
+        g, me = self.geoms[t], self.me[t]
+        k = self.nparts[t]
+        if side == "out":
+            segs = [self.send[t].ptr + g.send_displs[p] * self.esz for p in range(k)]
+            segs[me] = self.recv[t].ptr + g.recv_displs[me] * self.esz
+            start, count, trailing = g.src_s, g.src_n, g.src_trailing
+        else:
+            segs = [self.recv[t].ptr + g.recv_displs[p] * self.esz for p in range(k)]
+            start, count, trailing = g.dst_s, g.dst_n, g.dst_trailing
+        arrays = (self._arr(C.c_void_p, segs), self._arr(C.c_int, start),
+                  self._arr(C.c_int, count))
+        m = A.SegMap(k, arrays[0], arrays[1], arrays[2], trailing)
+        m._keep = arrays
+        return m
+
+    def _execute_segmented(self, plan, data, min_, mout):
+        A.check(A.load().pfx_execute_segmented(
+            plan.handle, A.PFX_FORWARD, data, C.byref(min_) if min_ is not None else None,
+            C.byref(mout) if mout is not None else None, 1.0, self.stream),
+            "pfx_execute_segmented")
+
+    def forward_fused(self, padded, events=None):
+        """Stage 0 R2C storing into transition 0's segments, then per stage a
+        C2C loading from the previous transition's receive segments and
+        storing into the next transition's segments (the last stage stores
+        into `padded`)."""
+        mark = events or (lambda label: None)
+        mark("r2c")
+        self._execute_segmented(self.r2c, ptr(padded), None, self.segmap(0, "out"))
+        for t in range(self.D - 1):
+            stage = t + 1
+            mark(f"fft{stage}")
+            mout = self.segmap(stage, "out") if stage < self.D - 1 else None
+            self._execute_segmented(self.plans[stage], ptr(padded), self.segmap(t, "in"), mout)
+        mark("end")
+
     def fft_stage(self, stage, buf):
         base = ptr(buf)
         for i in range(self.leading[stage]):
             self.plans[stage].execute(A.PFX_FORWARD, base + i * self.slice[stage], self.stream)
 
     def forward_in_place(self, padded, events=None):
-        """Stage 0 R2C, then per transition pack / self copy / unpack / C2C.
-        `events`, if given, is called with a label before each step (timing)."""
+        """Stage 0 R2C, then per transition pack / self copy / unpack / C2C
+        (or the fused form, see forward_fused).  `events`, if given, is
+        called with a label before each step (timing)."""
+        if self.fused:
+            return self.forward_fused(padded, events)
         mark = events or (lambda label: None)
         mark("r2c")
         self.r2c.execute(A.PFX_FORWARD, padded, self.stream)

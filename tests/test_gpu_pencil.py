@@ -6,7 +6,11 @@ left on the device as seeded synthetic segments.
   the reference's loops applied to the GPU's own intermediate data;
 - the whole share transform is checked against the oracle pipeline
   (FP64 restated R2C / pack / unpack / C2C) on scaled-down grids, and against
-  the FP64 GPU path at the full 2048^3-on-4x4 share (512 x 512 x 2048).
+  the FP64 GPU path at the full 2048^3-on-4x4 share (512 x 512 x 2048);
+- the fused form (pack and unpack inside the transforms) is checked
+  bit-exact against the stepwise path: final output, every peer's send
+  segment and the rank's own receive segment, on 2D, 3D and 4D shares and
+  at the full share.
 """
 import math
 
@@ -145,7 +149,8 @@ def test_pencil_share_small_vs_oracle(coords, fast):
     G = [128, 128, 128]
     for prec, rdt, cdt, tol in (("f32", np.float32, np.complex64, 1e-6 * 21),
                                 ("f64", np.float64, np.complex128, 1e-13)):
-        share = PencilShareR2C(G, [4, 4], list(coords), precision=prec, fast=fast)
+        share = PencilShareR2C(G, [4, 4], list(coords), precision=prec, fast=fast,
+                               fused=False)
         real = filtered_noise(tuple(share.stage_shapes[0]), seed=31).astype(rdt)
         peers = _fill_peers(share, 500, cdt)
         got = _run_stepwise(share, real, rdt)
@@ -172,7 +177,7 @@ def test_config3_full_share():
     pinned to the oracle above at <= 1e-13)."""
     from parafaft_amd.pencil import PencilShareR2C
     G = [2048, 2048, 2048]
-    share32 = PencilShareR2C(G, [4, 4], [0, 0], precision="f32")
+    share32 = PencilShareR2C(G, [4, 4], [0, 0], precision="f32", fused=False)
     assert share32.stage_shapes == [[512, 512, 2048], [512, 2048, 257], [2048, 512, 257]]
     assert len(share32.geoms) == 2 and all(len(g.send_counts) == 4 for g in share32.geoms)
     rng = np.random.default_rng(77)
@@ -180,8 +185,107 @@ def test_config3_full_share():
     _fill_peers(share32, 900, np.complex64)
     got32 = _run_stepwise(share32, real, np.float32, check_bytes=True)
     del share32
-    share64 = PencilShareR2C(G, [4, 4], [0, 0], precision="f64")
+    share64 = PencilShareR2C(G, [4, 4], [0, 0], precision="f64", fused=False)
     _fill_peers(share64, 900, np.complex128)
     got64 = _run_stepwise(share64, real.astype(np.float64), np.float64, check_bytes=False)
     err = rel_l2(got32, got64)
     assert err <= 1e-6 * 11, err
+
+
+def _segments(share):
+    """Host bytes of every peer's send segment and of the rank's own receive
+    segment, per transition (what leaves the rank and what it keeps)."""
+    out = {}
+    for t, g in enumerate(share.geoms):
+        for p in range(len(g.send_counts)):
+            if p == share.me[t]:
+                addr, count = share.recv_segment(t, p)
+                out[(t, "self")] = _download(addr, count * share.esz)
+            else:
+                addr, count = share.send_segment(t, p)
+                out[(t, p)] = _download(addr, count * share.esz)
+    return out
+
+
+def _run_fused(share, real, rdt):
+    n = real.shape[-1]
+    h = n // 2 + 1
+    pad = np.zeros(real.shape[:-1] + (2 * h,), rdt)
+    pad[..., :n] = real
+    buf = np.zeros(share.get_required_output_size(), rdt)
+    buf[:pad.size] = pad.ravel()
+    d = to_dev(buf)
+    share.forward_in_place(d)
+    nbytes = math.prod(share.stage_shapes[share.D - 1]) * share.esz
+    return _download(d.data_ptr(), nbytes)
+
+
+FUSED_CASES = [
+    ([128, 128, 128], [4, 4], [3, 1]),
+    ([128, 128, 128], [4, 4], [0, 2]),
+    ([96, 80, 64], [2, 3], [1, 2]),
+    ([64, 256], [4], [1]),
+    ([32, 16, 32, 64], [2, 2, 2], [1, 0, 1]),
+]
+
+
+@pytest.mark.parametrize("case", range(len(FUSED_CASES)))
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_fused_exchange_bit_exact(case, prec):
+    from parafaft_amd.pencil import PencilShareR2C
+    G, dims, coords = FUSED_CASES[case]
+    rdt, cdt = (np.float32, np.complex64) if prec == "f32" else (np.float64, np.complex128)
+    ref_share = PencilShareR2C(G, dims, coords, precision=prec, fused=False)
+    fused = PencilShareR2C(G, dims, coords, precision=prec)
+    assert fused.fused and fused.scratch is None
+    real = filtered_noise(tuple(ref_share.stage_shapes[0]), seed=41 + case).astype(rdt)
+    peers = _fill_peers(ref_share, 700, cdt)
+    _fill_peers(fused, 700, cdt)
+    want = _run_stepwise(ref_share, real, rdt, check_bytes=False)
+    got = _run_fused(fused, real, rdt)
+    a, b = _segments(ref_share), _segments(fused)
+    assert a.keys() == b.keys()
+    for k in a:
+        assert np.array_equal(a[k], b[k]), f"segment {k} differs"
+    assert np.array_equal(got, want.view(np.uint8).ravel()), "fused output differs"
+    if prec == "f64":
+        ref = _oracle_share(real.astype(np.float64), ref_share, peers)
+        assert rel_l2(got.view(cdt).reshape(ref.shape), ref) <= 1e-13
+
+
+def test_fused_full_share_bit_exact():
+    """The 512 x 512 x 2048 share of 2048^3 on 4x4, FP32: fused vs stepwise."""
+    from parafaft_amd.pencil import PencilShareR2C
+    G = [2048, 2048, 2048]
+    rng = np.random.default_rng(78)
+    real = rng.standard_normal((512, 512, 2048)).astype(np.float32)
+    ref_share = PencilShareR2C(G, [4, 4], [1, 2], precision="f32", fused=False)
+    _fill_peers(ref_share, 910, np.complex64)
+    want = _run_stepwise(ref_share, real, np.float32, check_bytes=False).view(np.uint8).ravel()
+    segs_want = _segments(ref_share)
+    del ref_share
+    fused = PencilShareR2C(G, [4, 4], [1, 2], precision="f32")
+    _fill_peers(fused, 910, np.complex64)
+    got = _run_fused(fused, real, np.float32)
+    assert np.array_equal(got, want)
+    segs = _segments(fused)
+    for k in segs_want:
+        assert np.array_equal(segs[k], segs_want[k]), f"segment {k} differs"
+
+
+def test_fused_rejects_bad_maps():
+    """Host-side checks of pfx_execute_segmented: blocks must tile the axis."""
+    import ctypes as C
+    from parafaft_amd import _capi as A
+    from parafaft_amd.backend import Plan
+    plan = Plan.c2c(A.PFX_F32, 64, 4, 1, 64)
+    d = torch().zeros(64 * 4 * 2, dtype=torch().float32, device="cuda")
+    segs = (C.c_void_p * 2)(d.data_ptr(), d.data_ptr())
+    m = A.SegMap(2, segs, (C.c_int * 2)(0, 30), (C.c_int * 2)(30, 30), 1)
+    rc = A.load().pfx_execute_segmented(plan.handle, A.PFX_FORWARD, d.data_ptr(), C.byref(m),
+                                        None, 1.0, None)
+    assert rc == A.PFX_ERR_INVALID_ARG
+    m = A.SegMap(2, segs, (C.c_int * 2)(0, 33), (C.c_int * 2)(32, 32), 1)
+    rc = A.load().pfx_execute_segmented(plan.handle, A.PFX_FORWARD, d.data_ptr(), C.byref(m),
+                                        None, 1.0, None)
+    assert rc == A.PFX_ERR_INVALID_ARG
