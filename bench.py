@@ -149,21 +149,31 @@ def bench_batch(args, torch, A):
     stream = torch.cuda.Stream()
     sp = stream.cuda_stream
     b = Batch("f32", [(k, s, x.data_ptr() + o * 8) for (k, s), o in zip(items, offs)])
+    flag = {"estimate": A.PFX_PLAN_ESTIMATE, "measure": A.PFX_PLAN_MEASURE,
+            "patient": A.PFX_PLAN_PATIENT}[args.plan]
+    if flag != A.PFX_PLAN_ESTIMATE:
+        t0 = time.perf_counter()
+        b.tune(flag, sp)          # every distinct stage geometry timed once
+        log(f"[bench] batch planning ({args.plan}): {time.perf_counter() - t0:.1f} s")
+        torch.cuda.synchronize()
+        x.copy_(x0)               # planning used the buffers as scratch
+        torch.cuda.synchronize()
     grouped, each = b.launches(A.PFX_FORWARD)
     points = sum(math.prod(s) for _, s in items)
     flops = sum((5.0 if k == "c2c" else 2.5) * math.prod(s) * math.log2(math.prod(s))
                 for k, s in items)
     res = {}
 
-    def timed(label, fn):
+    def timed(label, fn, steps=None):
+        steps = steps or args.steps
         for _ in range(args.warmup):
             fn()
         stream.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             fn()
         stream.synchronize()
-        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        ms = (time.perf_counter() - t0) * 1e3 / steps
         res[label] = {"ms": round(ms, 3), "gflops": round(flops / (ms * 1e-3) / 1e9, 1),
                       "gpoints_per_s": round(points / (ms * 1e-3) / 1e9, 3)}
         log(f"[bench] {label}: {ms:.3f} ms")
@@ -174,6 +184,39 @@ def bench_batch(args, torch, A):
     timed("grouped_graph", lambda: graph.launch(sp))
     graph_each = Graph(sp, lambda: b.execute_each(A.PFX_FORWARD, sp))
     timed("each_graph", lambda: graph_each.launch(sp))
+    del graph_each
+    # The reference's own pattern: one ParaFaFT object per shape, driven as its
+    # drivers drive a backend (per-slice middle-axis launches and the one-rank
+    # exchange copies, parafaft_c2c.hpp:539-602, 866-897), on the same kernels;
+    # replayed as one graph so host overhead is not counted.
+    if not args.no_reference_driver:
+        from parafaft_amd.parafaft import ParaFaFT_C2C, ParaFaFT_R2C
+        from parafaft_amd.backend import FFTPlanFlag
+        pf = FFTPlanFlag.Estimate if flag == A.PFX_PLAN_ESTIMATE else FFTPlanFlag.Measure
+        drivers = []
+        for (k, s), o in zip(items, offs):
+            if k == "c2c":
+                drivers.append((ParaFaFT_C2C(list(s), "f32", plan_flag=pf, fast=False,
+                                             stream=sp), "c2c", x.data_ptr() + o * 8))
+            else:
+                drivers.append((ParaFaFT_R2C(list(s), "f32", plan_flag=pf, fast=False,
+                                             stream=sp), "r2c", x.data_ptr() + o * 8))
+        torch.cuda.synchronize()
+        x.copy_(x0)
+        torch.cuda.synchronize()
+
+        def run_drivers():
+            for drv, kind, addr in drivers:
+                if kind == "c2c":
+                    drv.forward(addr, sync=False)
+                else:
+                    drv.forward_in_place(addr, sync=False)
+        t0 = time.perf_counter()
+        graph_ref = Graph(sp, run_drivers)
+        log(f"[bench] reference-pattern graph captured in {time.perf_counter() - t0:.1f} s")
+        timed("reference_driver_per_shape_graph", lambda: graph_ref.launch(sp),
+              steps=min(args.steps, 5))
+        del graph_ref, drivers
     # accuracy: one grouped forward on fresh data vs the FP64 batch on the same input
     acc = None
     if not args.no_check:
@@ -205,6 +248,10 @@ def bench_batch(args, torch, A):
                                          res["grouped"]["ms"], 3),
         "speedup_graph_vs_each": round(res["one_launch_per_transform_stage"]["ms"] /
                                        res["grouped_graph"]["ms"], 3),
+        "speedup_graph_vs_reference_driver": (
+            round(res["reference_driver_per_shape_graph"]["ms"] / res["grouped_graph"]["ms"], 3)
+            if "reference_driver_per_shape_graph" in res else None),
+        "planning": args.plan,
         "accuracy": acc}), flush=True)
 
 
@@ -223,6 +270,8 @@ def main():
     ap.add_argument("--workload", default="cube", choices=["cube", "pencil", "batch"],
                     help="cube: the 1024^3 C2C headline; pencil: one rank's share of a "
                          "2048^3 R2C on a 4x4 grid with on-device pack/unpack")
+    ap.add_argument("--no-reference-driver", action="store_true",
+                    help="batch workload: skip the one-ParaFaFT-object-per-shape baseline")
     ap.add_argument("--unfused", action="store_true",
                     help="pencil workload: separate pack / self copy / unpack launches")
     ap.add_argument("--calibrate-copy", action="store_true",

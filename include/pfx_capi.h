@@ -293,6 +293,17 @@ typedef struct pfx_batch_t *pfx_batch;
 int pfx_batch_create(pfx_batch *batch, int precision, int count, const pfx_batch_desc *descs);
 
 /* All transforms, grouped launches (stage level by stage level). */
+/*
+ * Plan every transform of the batch with `flag` (pfx_plan_tune on each stage,
+ * each distinct geometry timed once thanks to the tuning cache), using each
+ * transform's own buffer as the planning scratch, then rebuild the grouped
+ * schedule with the chosen kernels.  The FFTW_MEASURE analogue of
+ * one-plan-per-shape (fft_backend_fftw.hpp:743-754); buffer contents are
+ * lost, as with FFTW_MEASURE.  No counterpart in the reference's batch API,
+ * which has none.
+ */
+int pfx_batch_tune(pfx_batch batch, int flag, pfx_stream stream);
+
 int pfx_batch_execute(pfx_batch batch, int direction, pfx_stream stream);
 
 /* All transforms, one launch per transform stage (the ungrouped baseline). */

@@ -81,6 +81,7 @@ SIGNATURES = [
     ("pfx_ndplan_stage", _i, [_vp, _i, _vpp]),
     ("pfx_ndplan_tune", _i, [_vp, _i, _vp, _vp]),
     ("pfx_batch_create", _i, [_vpp, _i, _i, _vp]),
+    ("pfx_batch_tune", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute_each", _i, [_vp, _i, _vp]),
     ("pfx_batch_launches", _i, [_vp, _i, _ip, _ip]),

@@ -29,6 +29,11 @@ class Batch:
         self.handle = h.value
         self.count = len(items)
 
+    def tune(self, flag=A.PFX_PLAN_MEASURE, stream=None):
+        """Plan every transform with `flag` on its own buffer (contents are
+        lost) and regroup the launches around the chosen kernels."""
+        A.check(A.load().pfx_batch_tune(self.handle, flag, ptr(stream)), "pfx_batch_tune")
+
     def execute(self, direction, stream=None):
         A.check(A.load().pfx_batch_execute(self.handle, direction, ptr(stream)),
                 "pfx_batch_execute")

@@ -342,6 +342,7 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
   const bool real = p->kind != PFX_C2C;
   if (p->kernel) {
     StockhamArgs a;
+    a.src = nullptr;
     a.data = data;
     a.tw = p->tw;
     a.ptw = p->ptw;
@@ -423,6 +424,7 @@ int pfx_execute_segmented(pfx_plan p, int direction, void *data, const pfx_segma
   if ((!in || !out) && !data) return fail(PFX_ERR_INVALID_ARG, "data is null");
   const bool real = p->kind != PFX_C2C;
   StockhamArgs a;
+  a.src = nullptr;
   a.data = data;
   a.tw = p->tw;
   a.ptw = p->ptw;
@@ -440,6 +442,19 @@ int pfx_execute_segmented(pfx_plan p, int direction, void *data, const pfx_segma
   if (in) {
     int rc = to_segmap(in, (int)p->length, total, &a.in_map);
     if (rc) return rc;
+    // Segments back to back with one lead per block (the receive side of an
+    // exchange that gathers the outermost axis) are the plan's own strided
+    // layout: load plainly from the first segment, out of place.
+    const size_t esz = p->precision == PFX_F64 ? 16 : 8;
+    bool plain = total <= (int64_t)in->trailing && p->stride == (int64_t)in->trailing &&
+                 p->dist == 1 && (p->outer == 1 || p->dist_outer == p->inner);
+    for (int q = 0; plain && q < in->nparts; ++q)
+      plain = static_cast<const char *>(in->segments[q]) ==
+              static_cast<const char *>(in->segments[0]) + (size_t)in->start[q] * in->trailing * esz;
+    if (plain) {
+      a.src = in->segments[0];
+      a.in_seg = 0;
+    }
   }
   if (out) {
     int rc = to_segmap(out, real ? (int)p->length / 2 + 1 : (int)p->length, total, &a.out_map);
@@ -852,6 +867,7 @@ struct Level {
 StockhamArgs job_args(pfx_plan p, void *data, int direction) {
   const bool real = p->kind != PFX_C2C;
   StockhamArgs a;
+  a.src = nullptr;
   a.data = data;
   a.tw = p->tw;
   a.ptw = p->ptw;
@@ -884,44 +900,28 @@ struct pfx_graph_t {
 
 extern "C" {
 
-int pfx_batch_destroy(pfx_batch b) {
-  if (!b) return PFX_SUCCESS;
-  for (auto &dir : b->levels)
+}  // extern "C"
+
+namespace {
+
+void free_levels(pfx_batch b) {
+  for (auto &dir : b->levels) {
     for (auto &lv : dir)
       for (auto &g : lv.groups) {
-        if (g.d_jobs) hipFree(g.d_jobs);
-        if (g.d_start) hipFree(g.d_start);
+        if (g.d_jobs) (void)hipFree(g.d_jobs);
+        if (g.d_start) (void)hipFree(g.d_start);
       }
-  for (pfx_ndplan p : b->plans) pfx_ndplan_destroy(p);
-  delete b;
-  return PFX_SUCCESS;
+    dir.clear();
+  }
 }
 
-int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_desc *descs) {
-  if (!out) return fail(PFX_ERR_INVALID_ARG, "batch pointer is null");
-  *out = nullptr;
-  if (count < 0 || (count > 0 && !descs)) return fail(PFX_ERR_INVALID_ARG, "bad descriptor list");
-  pfx_batch b = new (std::nothrow) pfx_batch_t;
-  if (!b) return fail(PFX_ERR_ALLOC, "out of host memory");
-  b->precision = precision;
+// Groups every transform's stage work by level and kernel variant, with
+// device job tables (each job starting at a multiple of 8 workgroups).
+int build_levels(pfx_batch b) {
+  free_levels(b);
   int maxd = 0;
-  for (int i = 0; i < count; ++i) {
-    const pfx_batch_desc &d = descs[i];
-    if (d.ndim < 1 || d.ndim > 4) {
-      pfx_batch_destroy(b);
-      return fail(PFX_ERR_INVALID_ARG, "transform %d: ndim must be 1..4", i);
-    }
-    pfx_ndplan nd = nullptr;
-    int rc = pfx_ndplan_create(&nd, precision, d.kind, d.ndim, d.shape);
-    if (rc != PFX_SUCCESS) {
-      std::string msg = g_last_error;
-      pfx_batch_destroy(b);
-      return fail(rc, "transform %d: %s", i, msg.c_str());
-    }
-    b->plans.push_back(nd);
-    b->data.push_back(d.data);
-    if (d.ndim > maxd) maxd = d.ndim;
-  }
+  for (pfx_ndplan nd : b->plans) maxd = std::max(maxd, nd->ndim);
+  const int count = (int)b->plans.size();
   for (int dir = 0; dir < 2; ++dir) {
     b->levels[dir].resize(maxd);
     for (int i = 0; i < count; ++i) {
@@ -946,7 +946,6 @@ int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_d
         g->jobs.push_back(job_args(pl, b->data[i], dir));
       }
     }
-    // workgroup ranges, each starting at a multiple of 8 (XCD-aware order)
     for (auto &lv : b->levels[dir])
       for (auto &g : lv.groups) {
         int64_t pos = 0;
@@ -956,10 +955,7 @@ int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_d
           pos += (tiles + 7) / 8 * 8;
         }
         g.start.push_back(pos);
-        if (pos > 0x7fffffff) {
-          pfx_batch_destroy(b);
-          return fail(PFX_ERR_UNSUPPORTED, "grouped launch exceeds the grid limit");
-        }
+        if (pos > 0x7fffffff) return fail(PFX_ERR_UNSUPPORTED, "grouped launch exceeds the grid limit");
         hipError_t e = hipMalloc(&g.d_jobs, g.jobs.size() * sizeof(StockhamArgs));
         if (e == hipSuccess) e = hipMalloc(&g.d_start, g.start.size() * sizeof(int64_t));
         if (e == hipSuccess)
@@ -968,14 +964,67 @@ int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_d
         if (e == hipSuccess)
           e = hipMemcpy(g.d_start, g.start.data(), g.start.size() * sizeof(int64_t),
                         hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-          pfx_batch_destroy(b);
-          return hip_fail(e, "batch job tables");
-        }
+        if (e != hipSuccess) return hip_fail(e, "batch job tables");
       }
+  }
+  return PFX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfx_batch_destroy(pfx_batch b) {
+  if (!b) return PFX_SUCCESS;
+  free_levels(b);
+  for (pfx_ndplan p : b->plans) pfx_ndplan_destroy(p);
+  delete b;
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_desc *descs) {
+  if (!out) return fail(PFX_ERR_INVALID_ARG, "batch pointer is null");
+  *out = nullptr;
+  if (count < 0 || (count > 0 && !descs)) return fail(PFX_ERR_INVALID_ARG, "bad descriptor list");
+  pfx_batch b = new (std::nothrow) pfx_batch_t;
+  if (!b) return fail(PFX_ERR_ALLOC, "out of host memory");
+  b->precision = precision;
+  for (int i = 0; i < count; ++i) {
+    const pfx_batch_desc &d = descs[i];
+    if (d.ndim < 1 || d.ndim > 4) {
+      pfx_batch_destroy(b);
+      return fail(PFX_ERR_INVALID_ARG, "transform %d: ndim must be 1..4", i);
+    }
+    pfx_ndplan nd = nullptr;
+    int rc = pfx_ndplan_create(&nd, precision, d.kind, d.ndim, d.shape);
+    if (rc != PFX_SUCCESS) {
+      std::string msg = g_last_error;
+      pfx_batch_destroy(b);
+      return fail(rc, "transform %d: %s", i, msg.c_str());
+    }
+    b->plans.push_back(nd);
+    b->data.push_back(d.data);
+  }
+  int rc = build_levels(b);
+  if (rc != PFX_SUCCESS) {
+    std::string msg = g_last_error;
+    pfx_batch_destroy(b);
+    return fail(rc, "%s", msg.c_str());
   }
   *out = b;
   return PFX_SUCCESS;
+}
+
+int pfx_batch_tune(pfx_batch b, int flag, pfx_stream stream) {
+  if (!b) return fail(PFX_ERR_INVALID_ARG, "batch is null");
+  for (size_t i = 0; i < b->plans.size(); ++i) {
+    // the transform's own buffer is the planning scratch (its contents are lost)
+    int rc = pfx_ndplan_tune(b->plans[i], flag, b->data[i], stream);
+    if (rc != PFX_SUCCESS) return rc;
+  }
+  hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return hip_fail(e, "planning");
+  return build_levels(b);
 }
 
 int pfx_batch_execute(pfx_batch b, int direction, pfx_stream stream) {

@@ -41,21 +41,21 @@ void launch_c2c(const StockhamArgs &a, hipStream_t s) {
   else
     c2c_kernel<T, TPT, C, COL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
 void launch_c2c_wide(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
   if (a.in_seg || a.out_seg)
-    c2c_wide_kernel<T, TPT, C, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+    c2c_wide_kernel<T, TPT, C, ROWL, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
   else
-    c2c_wide_kernel<T, TPT, C, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+    c2c_wide_kernel<T, TPT, C, ROWL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, bool COL, int... Rs>
 void group_c2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
   c2c_group_kernel<T, TPT, C, COL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
 }
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
 void group_c2c_wide(const GroupArgs &g, int64_t blocks, hipStream_t s) {
-  c2c_wide_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
+  c2c_wide_group_kernel<T, TPT, C, ROWL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
 }
 template <typename T, int TPT, int C, int... Rs>
 void group_r2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {

@@ -79,6 +79,8 @@ __device__ __forceinline__ V2<T>* seg_addr(KSegMap* m, uint32_t lead, int64_t tr
 
 struct StockhamArgs {
   void* data;            // complex view of the buffer (R2C/C2R: padded rows)
+  const void* src;       // out of place: first-pass loads come from here, at
+                         // the same offsets (nullptr: in place, from data)
   const void* tw;        // W_L^i, i in [0, L): L = N (C2C) or 2N (R2C/C2R)
   const void* ptw;       // per-pass table of the LDS FFT: entry m*Ns + kk - 1
                          // holds W_{Ns R}^{kk m} for the pass with span Ns
@@ -179,17 +181,19 @@ __device__ inline void stockham_pass(int j, const V2<T>* __restrict__ stw, LoadF
 // instance, so the plain kernels carry none of its addressing).
 template <typename T, bool SEG = false> struct GlobalPtr {
   V2<T>* p;
+  const V2<T>* pin;          // load side (== p in place)
   int64_t stride;
   bool swap_io;
   bool valid;
   T scale;
   bool use_scale;
-  __device__ inline V2<T> load(int e) const { return p[(int64_t)e * stride]; }
+  __device__ inline V2<T> load(int e) const { return pin[(int64_t)e * stride]; }
   __device__ inline void store(int e, V2<T> v) const { p[(int64_t)e * stride] = v; }
 };
 
 template <typename T> struct GlobalPtr<T, true> {
   V2<T>* p;
+  const V2<T>* pin;
   int64_t stride;
   bool swap_io;
   bool valid;
@@ -201,7 +205,7 @@ template <typename T> struct GlobalPtr<T, true> {
   int64_t in_trail, out_trail;
   __device__ inline V2<T> load(int e) const {
     if (in_map) return *seg_addr<T>(in_map, in_lead, in_trail, e);
-    return p[(int64_t)e * stride];
+    return pin[(int64_t)e * stride];
   }
   __device__ inline void store(int e, V2<T> v) const {
     if (out_map) *seg_addr<T>(out_map, out_lead, out_trail, e) = v;
@@ -218,11 +222,15 @@ __device__ inline int64_t xform_offset(int64_t b, const StockhamArgs& a) {
 template <typename T, bool SEG>
 __device__ inline void init_global(GlobalPtr<T, SEG>& g, const StockhamArgs& a, int64_t b) {
   g.valid = b < a.total;
+  const V2<T>* in_base = static_cast<const V2<T>*>(a.src ? a.src : a.data);
   if constexpr (!SEG) {
-    g.p = static_cast<V2<T>*>(a.data) + (g.valid ? xform_offset(b, a) : 0);
+    const int64_t off = g.valid ? xform_offset(b, a) : 0;
+    g.p = static_cast<V2<T>*>(a.data) + off;
+    g.pin = in_base + off;
   } else {
-    g.p = static_cast<V2<T>*>(a.data) +
-          (g.valid && !(a.in_seg && a.out_seg) ? xform_offset(b, a) : 0);
+    const int64_t off = g.valid && !(a.in_seg && a.out_seg) ? xform_offset(b, a) : 0;
+    g.p = static_cast<V2<T>*>(a.data) + off;
+    g.pin = in_base + off;
     // Only single-plan kernels run segmented, and their one argument (the
     // StockhamArgs) sits at offset 0 of the kernel-argument segment; taking
     // the address of the parameter itself would copy it to scratch.
@@ -466,23 +474,41 @@ __global__ void __launch_bounds__(TPT * C) c2r_kernel(StockhamArgs a) {
 namespace pfx {
 
 // ---------------------------------------------------------------------------
-// Wide batch-contiguous C2C: C transforms per workgroup (C * 8 bytes of every
-// row in one run, 256 B for C = 32 complex<float>), each thread holding
-// E = N / TPT elements in registers across all passes.  The LDS exchange
-// between passes moves one component at a time (re, then im), so the tile
-// needs N * C scalars of LDS rather than N * C complex values.
+// Wide C2C: each thread holds E = N / TPT elements of its transform in
+// registers across all passes, and the LDS exchange between passes moves one
+// component at a time (re, then im), so a tile needs N * C scalars of LDS
+// rather than N * C complex values.  Two layouts:
+//  - batch-contiguous (ROWL = false): the C transforms of a workgroup are
+//    adjacent lanes, so every element index is one C * 8-byte run in HBM
+//    (256 B for C = 32 complex<float>); per-pass twiddles staged in LDS;
+//  - row (ROWL = true): the TPT threads of a transform are adjacent lanes,
+//    so first-pass loads and last-pass stores are contiguous runs of the
+//    row; twiddles are read from the (cached) global table, which keeps the
+//    workgroup's LDS at N * C scalars and several workgroups per CU.
 // Requires every radix to divide E (all passes are exact).
 // ---------------------------------------------------------------------------
-template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns, int... Rs>
-struct WidePasses;
-
-template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns>
-struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns> {
-  template <class G> __device__ static __forceinline__ void run(int, int, V2<T>*, T*, const V2<T>*, const G&) {}
+template <int N, int C, bool ROWL> struct WideLayout {
+  // row: one pad slot every 16 entries keeps the stride-R writes of a pass
+  // with Ns = 1 off a single bank
+  static constexpr int NP = N + N / 16 + 1;
+  static constexpr int size = ROWL ? NP * C : N * C;
+  __device__ static inline int at(int e, int c) {
+    if constexpr (ROWL) return c * NP + e + (e >> 4);
+    else return e * C + c;
+  }
 };
 
-template <typename T, int N, int TPT, int C, int E, int TWS, int PI, int Ns, int R, int... Rest>
-struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
+template <typename T, int N, int TPT, class WL, int E, int PI, int Ns, int... Rs>
+struct WidePasses;
+
+template <typename T, int N, int TPT, class WL, int E, int PI, int Ns>
+struct WidePasses<T, N, TPT, WL, E, PI, Ns> {
+  template <class G>
+  __device__ static __forceinline__ void run(int, int, V2<T>*, T*, const V2<T>*, const G&) {}
+};
+
+template <typename T, int N, int TPT, class WL, int E, int PI, int Ns, int R, int... Rest>
+struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
   template <class G>
   __device__ static __forceinline__ void run(int j, int c, V2<T>* v, T* lds, const V2<T>* tw, const G& g) {
@@ -527,7 +553,7 @@ struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
           const int ob = (jj / Ns) * Ns * R + jj % Ns;
 #pragma unroll
           for (int m = 0; m < R; ++m)
-            lds[(ob + m * Ns) * C + c] = comp ? v[q * R + m].y : v[q * R + m].x;
+            lds[WL::at(ob + m * Ns, c)] = comp ? v[q * R + m].y : v[q * R + m].x;
         }
         __syncthreads();
 #pragma unroll
@@ -535,30 +561,40 @@ struct WidePasses<T, N, TPT, C, E, TWS, PI, Ns, R, Rest...> {
           const int jj = j + q * TPT;
 #pragma unroll
           for (int m = 0; m < R2; ++m) {
-            const T s = lds[(jj + m * NB2) * C + c];
+            const T s = lds[WL::at(jj + m * NB2, c)];
             if (comp) v[q * R2 + m].y = s;
             else v[q * R2 + m].x = s;
           }
         }
         __syncthreads();
       }
-      WidePasses<T, N, TPT, C, E, TWS, PI + 1, Ns * R, Rest...>::run(j, c, v, lds, tw, g);
+      WidePasses<T, N, TPT, WL, E, PI + 1, Ns * R, Rest...>::run(j, c, v, lds, tw, g);
     }
   }
 };
 
-template <typename T, int TPT, int C, bool SEG, int... Rs>
+template <typename T, int TPT, int C, bool ROWL, bool SEG, int... Rs>
 __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   constexpr int E = N / TPT;
   constexpr int R0 = first_of<Rs...>();
   constexpr int Q0 = E / R0;
   constexpr int NB0 = N / R0;
-  __shared__ T lds[N * C];
-  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
-  const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
-  const int c = threadIdx.x % C, j = threadIdx.x / C;
-  tile = tile_order(tile, ntiles, a.xcd_remap);
+  using WL = WideLayout<N, C, ROWL>;
+  __shared__ T lds[WL::size];
+  const V2<T>* tw;
+  int c, j;
+  if constexpr (ROWL) {
+    tw = static_cast<const V2<T>*>(a.ptw);
+    j = threadIdx.x % TPT;
+    c = threadIdx.x / TPT;
+  } else {
+    __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+    tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
+    c = threadIdx.x % C;
+    j = threadIdx.x / C;
+    tile = tile_order(tile, ntiles, a.xcd_remap);
+  }
   const int64_t b = tile * C + c;
   GlobalPtr<T, SEG> g;
   init_global(g, a, b);
@@ -577,12 +613,12 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
       v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
     }
   }
-  WidePasses<T, N, TPT, C, E, 1, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
+  WidePasses<T, N, TPT, WL, E, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
 }
 
-template <typename T, int TPT, int C, bool SEG, int... Rs>
+template <typename T, int TPT, int C, bool ROWL, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_wide_kernel(StockhamArgs a) {
-  c2c_wide_body<T, TPT, C, SEG, Rs...>(a, blockIdx.x, gridDim.x);
+  c2c_wide_body<T, TPT, C, ROWL, SEG, Rs...>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -613,11 +649,12 @@ __global__ void __launch_bounds__(TPT * C) c2c_group_kernel(GroupArgs ga) {
   const int64_t s0 = ga.start[jb];
   c2c_body<T, TPT, C, COL, false, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
 }
-template <typename T, int TPT, int C, int... Rs>
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_wide_group_kernel(GroupArgs ga) {
   const int jb = find_job(ga, blockIdx.x);
   const int64_t s0 = ga.start[jb];
-  c2c_wide_body<T, TPT, C, false, Rs...>(ga.jobs[jb], blockIdx.x - s0, ga.start[jb + 1] - s0);
+  c2c_wide_body<T, TPT, C, ROWL, false, Rs...>(ga.jobs[jb], blockIdx.x - s0,
+                                              ga.start[jb + 1] - s0);
 }
 template <typename T, int TPT, int C, int... Rs>
 __global__ void __launch_bounds__(TPT * C) r2c_group_kernel(GroupArgs ga) {

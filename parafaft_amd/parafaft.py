@@ -122,12 +122,13 @@ class ParaFaFT_C2C:
             for i in range(leading):
                 self.backend.execute_stage(stage, direction, base + i * slice_bytes, scale)
 
-    def _run(self, data, forward: bool, scale: float):
+    def _run(self, data, forward: bool, scale: float, sync: bool = True):
         D = self.D
         if self.fast:
             self._nd.execute(A.PFX_FORWARD if forward else A.PFX_BACKWARD, data, scale,
                              self.backend.stream)
-            self.backend.sync()
+            if sync:
+                self.backend.sync()
             return
         nbytes = self.max_stage_size * self.esz
         if (D - 1) % 2 == 0:
@@ -146,16 +147,19 @@ class ParaFaFT_C2C:
                 # exchange_local on one rank: identity flat copy
                 self.backend.memcpy(dst, src, nbytes)
                 src, dst = dst, src
-        self.backend.sync()
+        if sync:
+            self.backend.sync()
 
-    def forward(self, data):
-        """parafaft_c2c.hpp:539-602 (unnormalised)."""
-        self._run(data, True, 1.0)
+    def forward(self, data, sync: bool = True):
+        """parafaft_c2c.hpp:539-602 (unnormalised).  ``sync=False`` leaves the
+        closing stream synchronisation (c2c:601) to the caller, e.g. while the
+        stream is being captured into a graph."""
+        self._run(data, True, 1.0, sync)
 
-    def backward(self, data, scale: float = 1.0):
+    def backward(self, data, scale: float = 1.0, sync: bool = True):
         """parafaft_c2c.hpp:632-694; ``scale`` (default 1, the reference's
         unnormalised convention) is fused into the last pass when != 1."""
-        self._run(data, False, scale)
+        self._run(data, False, scale, sync)
 
 
 class ParaFaFT_R2C:
@@ -233,12 +237,13 @@ class ParaFaFT_R2C:
             for i in range(leading):
                 self.backend.execute_stage(stage, direction, base + i * slice_bytes, scale)
 
-    def forward_in_place(self, padded):
-        """parafaft_r2c.hpp:360-430."""
+    def forward_in_place(self, padded, sync: bool = True):
+        """parafaft_r2c.hpp:360-430 (``sync`` as in ParaFaFT_C2C.forward)."""
         D = self.D
         if self.fast:
             self._nd.execute(A.PFX_FORWARD, padded, 1.0, self.backend.stream)
-            self.backend.sync()
+            if sync:
+                self.backend.sync()
             return
         nbytes = self.max_complex_size * self.esz
         self.backend.execute_r2c_inplace(padded)
@@ -252,7 +257,8 @@ class ParaFaFT_R2C:
             self.backend.memcpy(dst, src, nbytes)  # exchange_local: identity
             self._perform_fft_on_buffer(stage, axis, FFTDirection.Forward, dst)
             src, dst = dst, src
-        self.backend.sync()
+        if sync:
+            self.backend.sync()
 
     def backward_in_place(self, padded, scale: float = 1.0):
         """parafaft_r2c.hpp:473-542 (unnormalised unless scale != 1)."""
