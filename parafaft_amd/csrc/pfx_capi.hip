@@ -5,6 +5,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <map>
 #include <mutex>
 #include <string>
@@ -890,7 +891,7 @@ struct pfx_batch_t {
   int precision = PFX_F32;
   std::vector<pfx_ndplan> plans;
   std::vector<void *> data;
-  std::vector<Level> levels[2];  // [direction][level]
+  std::vector<Level> levels[2];  // [direction]: every chunk's stage levels, in order
 };
 
 struct pfx_graph_t {
@@ -915,35 +916,78 @@ void free_levels(pfx_batch b) {
   }
 }
 
-// Groups every transform's stage work by level and kernel variant, with
-// device job tables (each job starting at a multiple of 8 workgroups).
+// Bytes a transform occupies in HBM (its complex view).
+int64_t footprint(pfx_ndplan nd) {
+  int64_t elems = 1;
+  for (int k = 0; k + 1 < nd->ndim; ++k) elems *= nd->shape[k];
+  const int last = nd->shape[nd->ndim - 1];
+  elems *= nd->kind == PFX_C2C ? last : last / 2 + 1;
+  return elems * (nd->precision == PFX_F64 ? 16 : 8);
+}
+
+int64_t env_mb(const char *name, int64_t dflt) {
+  const char *v = getenv(name);
+  return v && *v ? (int64_t)atoll(v) : dflt;
+}
+
+// The schedule: transforms of at least kSmall bytes are packed, largest
+// first, into chunks of at most kChunk bytes, and each chunk runs all its
+// stages before the next starts, so the later passes of a chunk find its
+// data still in the 256 MB Infinity Cache (MALL).  Everything smaller forms
+// one chunk whose stages are grouped across all its transforms (those are
+// too small to fill the GPU one at a time).  Within a chunk, each stage level
+// is one grouped launch per kernel variant, with device job tables (each job
+// starting at a multiple of 8 workgroups for the XCD-aware order).
 int build_levels(pfx_batch b) {
   free_levels(b);
-  int maxd = 0;
-  for (pfx_ndplan nd : b->plans) maxd = std::max(maxd, nd->ndim);
   const int count = (int)b->plans.size();
+  const int64_t chunk_bytes = env_mb("PFX_BATCH_CHUNK_MB", 192) << 20;
+  const int64_t small_bytes = env_mb("PFX_BATCH_SMALL_MB", 16) << 20;
+  std::vector<std::vector<int>> chunks;
+  std::vector<int> small, big;
+  for (int i = 0; i < count; ++i)
+    (footprint(b->plans[i]) >= small_bytes && chunk_bytes > 0 ? big : small).push_back(i);
+  std::stable_sort(big.begin(), big.end(), [&](int x, int y) {
+    return footprint(b->plans[x]) > footprint(b->plans[y]);
+  });
+  int64_t used = 0;
+  for (int i : big) {
+    const int64_t f = footprint(b->plans[i]);
+    if (chunks.empty() || used + f > chunk_bytes) {
+      chunks.emplace_back();
+      used = 0;
+    }
+    chunks.back().push_back(i);
+    used += f;
+  }
+  if (!small.empty()) chunks.push_back(small);
   for (int dir = 0; dir < 2; ++dir) {
-    b->levels[dir].resize(maxd);
-    for (int i = 0; i < count; ++i) {
-      pfx_ndplan nd = b->plans[i];
-      const int D = nd->ndim;
-      for (int k = 0; k < D; ++k) {
-        const int s = dir == PFX_FORWARD ? k : D - 1 - k;
-        pfx_plan pl = nd->stages[s];
-        if (nd->kind == PFX_R2C && s == 0 && dir == PFX_BACKWARD) pl = nd->stages[D];
-        Level &lv = b->levels[dir][k];
-        if (!pl->kernel || !pl->kernel->launch_group) {
-          lv.direct.push_back({pl, b->data[i]});
-          continue;
+    for (const std::vector<int> &chunk : chunks) {
+      int maxd = 0;
+      for (int i : chunk) maxd = std::max(maxd, b->plans[i]->ndim);
+      const size_t base = b->levels[dir].size();
+      b->levels[dir].resize(base + maxd);
+      for (int i : chunk) {
+        pfx_ndplan nd = b->plans[i];
+        const int D = nd->ndim;
+        for (int k = 0; k < D; ++k) {
+          const int st = dir == PFX_FORWARD ? k : D - 1 - k;
+          pfx_plan pl = nd->stages[st];
+          if (nd->kind == PFX_R2C && st == 0 && dir == PFX_BACKWARD) pl = nd->stages[D];
+          Level &lv = b->levels[dir][base + k];
+          if (!pl->kernel || !pl->kernel->launch_group) {
+            lv.direct.push_back({pl, b->data[i]});
+            continue;
+          }
+          Group *g = nullptr;
+          for (auto &gg : lv.groups)
+            if (gg.kernel == pl->kernel) g = &gg;
+          if (!g) {
+            lv.groups.push_back(Group{pl->kernel, {}, {}, nullptr, nullptr});
+            g = &lv.groups.back();
+          }
+          g->jobs.push_back(job_args(pl, b->data[i], dir));
         }
-        Group *g = nullptr;
-        for (auto &gg : lv.groups)
-          if (gg.kernel == pl->kernel) g = &gg;
-        if (!g) {
-          lv.groups.push_back(Group{pl->kernel, {}, {}, nullptr, nullptr});
-          g = &lv.groups.back();
-        }
-        g->jobs.push_back(job_args(pl, b->data[i], dir));
       }
     }
     for (auto &lv : b->levels[dir])

@@ -189,3 +189,34 @@ def test_batch_tuned_vs_oracle(prec):
         outs[mode] = to_host(d)
         _check(SMALL, outs[mode], offs, refs, prec, cdt)
     assert np.array_equal(outs["grouped"].view(np.uint8), outs["each"].view(np.uint8))
+
+
+def test_batch_chunked_schedule(monkeypatch):
+    """Cache-sized chunks (forced tiny here: every transform of 64 KiB or
+    more is chunked, 1 MiB per chunk): the chunked grouped schedule gives
+    the same bytes as one launch per stage, and matches the oracle."""
+    from parafaft_amd.batch import Batch
+    monkeypatch.setenv("PFX_BATCH_SMALL_MB", "0")
+    monkeypatch.setenv("PFX_BATCH_CHUNK_MB", "1")
+    prec, cdt, rdt = "f32", np.complex64, np.float32
+    offs, total = _layout(SMALL, cdt)
+    xs = _inputs(SMALL, 230)
+    host = _pack_inputs(SMALL, xs, offs, total, cdt, rdt)
+    refs = _refs(SMALL, [x.astype(cdt if k == "c2c" else rdt).astype(
+        np.complex128 if k == "c2c" else np.float64) for (k, _), x in zip(SMALL, xs)])
+    d = to_dev(host)
+    b = Batch(prec, [(k, s, d.data_ptr() + o * 8) for (k, s), o in zip(SMALL, offs)])
+    g, e = b.launches(0)
+    monkeypatch.delenv("PFX_BATCH_SMALL_MB")
+    monkeypatch.delenv("PFX_BATCH_CHUNK_MB")
+    b_one = Batch(prec, [(k, s, d.data_ptr() + o * 8) for (k, s), o in zip(SMALL, offs)])
+    g_one, _ = b_one.launches(0)
+    assert g > g_one   # chunking splits the levels
+    outs = {}
+    for mode in ("chunked", "each"):
+        d.copy_(torch().from_numpy(host).to("cuda"))
+        torch().cuda.synchronize()
+        (b.execute if mode == "chunked" else b.execute_each)(0)
+        outs[mode] = to_host(d)
+        _check(SMALL, outs[mode], offs, refs, prec, cdt)
+    assert np.array_equal(outs["chunked"].view(np.uint8), outs["each"].view(np.uint8))
