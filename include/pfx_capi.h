@@ -258,8 +258,23 @@ int pfx_ndplan_execute(pfx_ndplan plan, int direction, void *data, double scale,
 int pfx_ndplan_destroy(pfx_ndplan plan);
 
 /* Tunes every stage plan (pfx_plan_tune) on `scratch`, a device buffer of the
- * whole array (contents destroyed). */
+ * whole array (contents destroyed), and keeps the chunked schedule (below)
+ * only if it times faster. */
 int pfx_ndplan_tune(pfx_ndplan plan, int flag, void *scratch, pfx_stream stream);
+
+/*
+ * Chunked schedule: for ndim >= 3 arrays larger than twice the Infinity Cache
+ * budget (PFX_ND_CHUNK_MB, 256 MB by default), every stage but axis 0 runs on
+ * `planes` axis-0 planes at a time, so a chunk's later stages find its data
+ * in the cache; axis 0 then runs over the whole array.  Same kernels, same
+ * results.  `planes` divides n0; 0 = stage by stage.  No counterpart in the
+ * reference (its stage loop is parafaft_c2c.hpp:539-602).
+ */
+int pfx_ndplan_chunk(pfx_ndplan plan, int *planes);
+int pfx_ndplan_set_chunk(pfx_ndplan plan, int planes);
+/* Chunk plan of stage s (0..ndim-2; ndim-1 is the C2R of an R2C plan), applied
+ * at data + c * planes * (plane elements), for per-stage timing. */
+int pfx_ndplan_chunk_stage(pfx_ndplan plan, int stage, pfx_plan *out);
 
 /* Stage plan of axis-stage s (0 = last axis), for per-stage timing. */
 int pfx_ndplan_stage(pfx_ndplan plan, int stage, pfx_plan *out);

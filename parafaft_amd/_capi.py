@@ -80,6 +80,9 @@ SIGNATURES = [
     ("pfx_ndplan_destroy", _i, [_vp]),
     ("pfx_ndplan_stage", _i, [_vp, _i, _vpp]),
     ("pfx_ndplan_tune", _i, [_vp, _i, _vp, _vp]),
+    ("pfx_ndplan_chunk", _i, [_vp, _ip]),
+    ("pfx_ndplan_set_chunk", _i, [_vp, _i]),
+    ("pfx_ndplan_chunk_stage", _i, [_vp, _i, _vpp]),
     ("pfx_batch_create", _i, [_vpp, _i, _i, _vp]),
     ("pfx_batch_tune", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute", _i, [_vp, _i, _vp]),

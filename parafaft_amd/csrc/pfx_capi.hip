@@ -155,7 +155,13 @@ struct pfx_ndplan_t {
   int kind = PFX_C2C;
   int ndim = 0;
   std::vector<int> shape;
-  std::vector<pfx_plan> stages;  // stages[s] transforms axis ndim-1-s
+  std::vector<pfx_plan> stages;  // stages[s] transforms axis ndim-1-s (+ C2R for R2C)
+  // Chunked schedule: every stage except axis 0 runs on `chunk` axis-0 planes
+  // at a time (cstages: the same plans restricted to a chunk, + C2R for R2C),
+  // so a chunk's later stages find its data in the Infinity Cache.  0 = off.
+  int chunk = 0;
+  int64_t plane_elems = 0;        // complex elements per axis-0 plane
+  std::vector<pfx_plan> cstages;
 };
 
 extern "C" {
@@ -749,6 +755,51 @@ int pfx_scale(void *data, int64_t count, int precision, double s, pfx_stream str
 
 // ------------------------------------------------------ one-rank N-D path
 
+}  // extern "C"
+
+namespace {
+
+int64_t env_mb_nd(const char *name, int64_t dflt) {
+  const char *v = getenv(name);
+  return v && *v ? (int64_t)atoll(v) : dflt;
+}
+
+// Plans of stages 0..D-2 restricted to P axis-0 planes (+ the C2R on P planes).
+int make_chunk_plans(pfx_ndplan p, const std::vector<int64_t> &cs, int P) {
+  const int ndim = p->ndim;
+  const int prec = p->precision;
+  for (int s = 0; s < ndim - 1; ++s) {
+    const int axis = ndim - 1 - s;
+    int64_t lead = P, trail = 1;
+    for (int i = 1; i < axis; ++i) lead *= cs[i];
+    for (int i = axis + 1; i < ndim; ++i) trail *= cs[i];
+    pfx_plan pl = nullptr;
+    int rc;
+    if (axis == ndim - 1)
+      rc = p->kind == PFX_R2C
+               ? pfx_plan_create_r2c(&pl, prec, p->shape[axis], lead, 1, 2 * cs[axis])
+               : pfx_plan_create_c2c(&pl, prec, cs[axis], lead, 1, cs[axis]);
+    else
+      rc = pfx_plan_create_c2c_many(&pl, prec, cs[axis], trail, trail, 1, lead, cs[axis] * trail);
+    if (rc != PFX_SUCCESS) return rc;
+    p->cstages.push_back(pl);
+  }
+  if (p->kind == PFX_R2C) {
+    int64_t lead = P;
+    for (int i = 1; i < ndim - 1; ++i) lead *= cs[i];
+    pfx_plan pl = nullptr;
+    int rc = pfx_plan_create_c2r(&pl, prec, p->shape[ndim - 1], lead, 1, 2 * cs[ndim - 1]);
+    if (rc != PFX_SUCCESS) return rc;
+    p->cstages.push_back(pl);
+  }
+  p->chunk = P;
+  return PFX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
 int pfx_ndplan_create(pfx_ndplan *out, int precision, int kind, int ndim, const int *shape) {
   if (!out) return fail(PFX_ERR_INVALID_ARG, "plan pointer is null");
   *out = nullptr;
@@ -803,8 +854,60 @@ int pfx_ndplan_create(pfx_ndplan *out, int precision, int kind, int ndim, const 
     }
     p->stages.push_back(c2r);
   }
+  // Chunked schedule for arrays well beyond the Infinity Cache: the largest
+  // divisor P of n0 whose P planes fit the budget.
+  p->plane_elems = 1;
+  for (int i = 1; i < ndim; ++i) p->plane_elems *= cs[i];
+  const int64_t esz = base_prec(precision) ? 16 : 8;
+  const int64_t budget = env_mb_nd("PFX_ND_CHUNK_MB", 256) << 20;
+  const int64_t plane_bytes = p->plane_elems * esz;
+  int P = 0;
+  if (ndim >= 3 && budget > 0 && (int64_t)shape[0] * plane_bytes > 2 * budget)
+    for (int d = shape[0] - 1; d >= 1; --d)
+      if (shape[0] % d == 0 && d * plane_bytes <= budget) {
+        P = d;
+        break;
+      }
+  if (P > 0) {
+    int rc = make_chunk_plans(p, cs, P);
+    if (rc != PFX_SUCCESS) {
+      std::string msg = g_last_error;
+      pfx_ndplan_destroy(p);
+      return fail(rc, "chunk plans: %s", msg.c_str());
+    }
+  }
   *out = p;
   return PFX_SUCCESS;
+}
+
+int pfx_ndplan_chunk(pfx_ndplan p, int *planes) {
+  if (!p || !planes) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  *planes = p->chunk;
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_chunk_stage(pfx_ndplan p, int stage, pfx_plan *out) {
+  if (!p || !out) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  if (p->chunk == 0) return fail(PFX_ERR_INVALID_ARG, "the plan runs stage by stage");
+  if (stage < 0 || stage >= (int)p->cstages.size()) return fail(PFX_ERR_INVALID_ARG, "bad stage");
+  *out = p->cstages[stage];
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_set_chunk(pfx_ndplan p, int planes) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (planes == p->chunk) return PFX_SUCCESS;
+  if (planes != 0 && (p->ndim < 3 || planes < 1 || planes >= p->shape[0] ||
+                      p->shape[0] % planes != 0))
+    return fail(PFX_ERR_INVALID_ARG, "chunk must divide n0 = %d and be below it (ndim >= 3)",
+                p->ndim ? p->shape[0] : 0);
+  for (pfx_plan c : p->cstages) pfx_plan_destroy(c);
+  p->cstages.clear();
+  p->chunk = 0;
+  if (planes == 0) return PFX_SUCCESS;
+  std::vector<int64_t> cs(p->shape.begin(), p->shape.end());
+  if (p->kind == PFX_R2C) cs[p->ndim - 1] = p->shape[p->ndim - 1] / 2 + 1;
+  return make_chunk_plans(p, cs, planes);
 }
 
 int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pfx_stream stream) {
@@ -812,6 +915,34 @@ int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pf
   if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
     return fail(PFX_ERR_INVALID_ARG, "bad direction");
   const int D = p->ndim;
+  if (p->chunk > 0) {
+    const int64_t esz = base_prec(p->precision) ? 16 : 8;
+    const int nchunks = p->shape[0] / p->chunk;
+    const size_t chunk_bytes = (size_t)(p->chunk * p->plane_elems * esz);
+    auto run_chunks = [&](double last_scale) -> int {
+      for (int c = 0; c < nchunks; ++c) {
+        char *base = static_cast<char *>(data) + (size_t)c * chunk_bytes;
+        for (int k = 0; k < D - 1; ++k) {
+          // forward: stages 0..D-2; backward: D-2..0 (C2R last for R2C)
+          const int st = direction == PFX_FORWARD ? k : D - 2 - k;
+          pfx_plan pl = p->cstages[st];
+          if (p->kind == PFX_R2C && st == 0 && direction == PFX_BACKWARD) pl = p->cstages[D - 1];
+          const double sc = (k == D - 2) ? last_scale : 1.0;
+          int rc = pfx_execute_scaled(pl, direction, base, sc, stream);
+          if (rc != PFX_SUCCESS) return rc;
+        }
+      }
+      return PFX_SUCCESS;
+    };
+    if (direction == PFX_FORWARD) {
+      int rc = run_chunks(1.0);
+      if (rc != PFX_SUCCESS) return rc;
+      return pfx_execute_scaled(p->stages[D - 1], direction, data, scale, stream);
+    }
+    int rc = pfx_execute_scaled(p->stages[D - 1], direction, data, 1.0, stream);
+    if (rc != PFX_SUCCESS) return rc;
+    return run_chunks(scale);
+  }
   for (int k = 0; k < D; ++k) {
     const int s = direction == PFX_FORWARD ? k : D - 1 - k;
     pfx_plan pl = p->stages[s];
@@ -826,6 +957,7 @@ int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pf
 int pfx_ndplan_destroy(pfx_ndplan p) {
   if (!p) return PFX_SUCCESS;
   for (pfx_plan s : p->stages) pfx_plan_destroy(s);
+  for (pfx_plan s : p->cstages) pfx_plan_destroy(s);
   delete p;
   return PFX_SUCCESS;
 }
@@ -835,6 +967,35 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
   for (pfx_plan s : p->stages) {
     int rc = pfx_plan_tune(s, flag, scratch, stream);
     if (rc != PFX_SUCCESS) return rc;
+  }
+  for (pfx_plan s : p->cstages) {
+    int rc = pfx_plan_tune(s, flag, scratch, stream);
+    if (rc != PFX_SUCCESS) return rc;
+  }
+  if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE) {
+    // keep the chunked schedule only if it times faster than stage-by-stage
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t a, b;
+    PFX_HIP(hipEventCreate(&a), "hipEventCreate");
+    PFX_HIP(hipEventCreate(&b), "hipEventCreate");
+    const int reps = flag == PFX_PLAN_PATIENT ? 6 : 2;
+    float ms[2] = {0.f, 0.f};
+    const int chunk = p->chunk;
+    int rc = PFX_SUCCESS;
+    for (int mode = 0; mode < 2 && rc == PFX_SUCCESS; ++mode) {
+      p->chunk = mode ? chunk : 0;
+      rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);  // warm-up
+      hipEventRecord(a, s);
+      for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r)
+        rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);
+      hipEventRecord(b, s);
+      if (hipEventSynchronize(b) == hipSuccess) hipEventElapsedTime(&ms[mode], a, b);
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
+    p->chunk = chunk;
+    if (rc != PFX_SUCCESS) return rc;
+    if (ms[0] <= ms[1]) return pfx_ndplan_set_chunk(p, 0);
   }
   return PFX_SUCCESS;
 }
