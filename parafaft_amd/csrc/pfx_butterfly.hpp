@@ -13,23 +13,23 @@ template <> struct V2T<float> { using type = float2; };
 template <> struct V2T<double> { using type = double2; };
 template <typename T> using V2 = typename V2T<T>::type;
 
-template <typename T> __host__ __device__ inline V2<T> mk(T x, T y) { V2<T> r; r.x = x; r.y = y; return r; }
-template <typename T> __host__ __device__ inline V2<T> add(V2<T> a, V2<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
-template <typename T> __host__ __device__ inline V2<T> sub(V2<T> a, V2<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
-template <typename T> __host__ __device__ inline V2<T> cmul(V2<T> a, V2<T> b) {
+template <typename T> __host__ __device__ __forceinline__ V2<T> mk(T x, T y) { V2<T> r; r.x = x; r.y = y; return r; }
+template <typename T> __host__ __device__ __forceinline__ V2<T> add(V2<T> a, V2<T> b) { return mk<T>(a.x + b.x, a.y + b.y); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> sub(V2<T> a, V2<T> b) { return mk<T>(a.x - b.x, a.y - b.y); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> cmul(V2<T> a, V2<T> b) {
   return mk<T>(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 // a * conj(b)
-template <typename T> __host__ __device__ inline V2<T> cmulc(V2<T> a, V2<T> b) {
+template <typename T> __host__ __device__ __forceinline__ V2<T> cmulc(V2<T> a, V2<T> b) {
   return mk<T>(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
 }
-template <typename T> __host__ __device__ inline V2<T> conj(V2<T> a) { return mk<T>(a.x, -a.y); }
-template <typename T> __host__ __device__ inline V2<T> swp(V2<T> a) { return mk<T>(a.y, a.x); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> conj(V2<T> a) { return mk<T>(a.x, -a.y); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> swp(V2<T> a) { return mk<T>(a.y, a.x); }
 // -i * a
-template <typename T> __host__ __device__ inline V2<T> mul_mi(V2<T> a) { return mk<T>(a.y, -a.x); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> mul_mi(V2<T> a) { return mk<T>(a.y, -a.x); }
 // +i * a
-template <typename T> __host__ __device__ inline V2<T> mul_pi(V2<T> a) { return mk<T>(-a.y, a.x); }
-template <typename T> __host__ __device__ inline V2<T> scl(V2<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> mul_pi(V2<T> a) { return mk<T>(-a.y, a.x); }
+template <typename T> __host__ __device__ __forceinline__ V2<T> scl(V2<T> a, T s) { return mk<T>(a.x * s, a.y * s); }
 
 namespace k {
 constexpr double SQRT1_2 = 0.70710678118654752440084436210484904;
@@ -46,14 +46,14 @@ constexpr double COS_3PI16 = 0.83146961230254523707878837761790576;
 constexpr double SIN_3PI16 = 0.55557023301960222474283081394853287;
 }  // namespace k
 
-template <typename T> __host__ __device__ inline void bfly2(V2<T>* v) {
+template <typename T> __host__ __device__ __forceinline__ void bfly2(V2<T>* v) {
   V2<T> a = v[0], b = v[1];
   v[0] = add<T>(a, b);
   v[1] = sub<T>(a, b);
 }
 
 // in-place radix-4 on v[0], v[s], v[2s], v[3s] (natural order in and out)
-template <typename T, int S = 1> __host__ __device__ inline void bfly4s(V2<T>* v) {
+template <typename T, int S = 1> __host__ __device__ __forceinline__ void bfly4s(V2<T>* v) {
   V2<T> a0 = v[0], a1 = v[S], a2 = v[2 * S], a3 = v[3 * S];
   V2<T> t0 = add<T>(a0, a2), t1 = sub<T>(a0, a2);
   V2<T> t2 = add<T>(a1, a3), t3 = mul_mi<T>(sub<T>(a1, a3));
@@ -62,9 +62,9 @@ template <typename T, int S = 1> __host__ __device__ inline void bfly4s(V2<T>* v
   v[S] = add<T>(t1, t3);
   v[3 * S] = sub<T>(t1, t3);
 }
-template <typename T> __host__ __device__ inline void bfly4(V2<T>* v) { bfly4s<T, 1>(v); }
+template <typename T> __host__ __device__ __forceinline__ void bfly4(V2<T>* v) { bfly4s<T, 1>(v); }
 
-template <typename T> __host__ __device__ inline void bfly3(V2<T>* v) {
+template <typename T> __host__ __device__ __forceinline__ void bfly3(V2<T>* v) {
   const T h = T(k::SQRT3_2);
   V2<T> a = v[0], b = v[1], c = v[2];
   V2<T> t1 = add<T>(b, c);
@@ -75,7 +75,7 @@ template <typename T> __host__ __device__ inline void bfly3(V2<T>* v) {
   v[2] = add<T>(t2, mul_pi<T>(t3));
 }
 
-template <typename T> __host__ __device__ inline void bfly5(V2<T>* v) {
+template <typename T> __host__ __device__ __forceinline__ void bfly5(V2<T>* v) {
   const T c1 = T(k::C5_1), c2 = T(k::C5_2), s1 = T(k::S5_1), s2 = T(k::S5_2);
   V2<T> a0 = v[0];
   V2<T> b1 = add<T>(v[1], v[4]), b2 = add<T>(v[2], v[3]);
@@ -92,7 +92,7 @@ template <typename T> __host__ __device__ inline void bfly5(V2<T>* v) {
 }
 
 // radix-8 = 2 x radix-4: evens / odds, twiddle W8^k on the odds, combine
-template <typename T> __host__ __device__ inline void bfly8(V2<T>* v) {
+template <typename T> __host__ __device__ __forceinline__ void bfly8(V2<T>* v) {
   const T r = T(k::SQRT1_2);
   V2<T> e[4] = {v[0], v[2], v[4], v[6]};
   V2<T> o[4] = {v[1], v[3], v[5], v[7]};
@@ -110,7 +110,7 @@ template <typename T> __host__ __device__ inline void bfly8(V2<T>* v) {
 }
 
 // radix-16 as 4 x 4 (decimation in time, n = 4 n1 + n2, k = k1 + 4 k2)
-template <typename T> __host__ __device__ inline void bfly16(V2<T>* v) {
+template <typename T> __host__ __device__ __forceinline__ void bfly16(V2<T>* v) {
   const T c = T(k::COS_PI8), s = T(k::SIN_PI8), r = T(k::SQRT1_2);
   // step 1: for each n2, DFT4 over n1 of v[4 n1 + n2]  -> F[n2][k1] stored at v[4 k1 + n2]
 #pragma unroll
@@ -142,7 +142,7 @@ template <typename T> __host__ __device__ inline void bfly16(V2<T>* v) {
 
 // radix-32 = 2 x radix-16 (decimation in time): X[k] = E[k] + W32^k O[k],
 // X[k + 16] = E[k] - W32^k O[k], W32^k = cos(pi k/16) - i sin(pi k/16).
-template <typename T> __host__ __device__ inline void bfly32(V2<T>* v) {
+template <typename T> __host__ __device__ __forceinline__ void bfly32(V2<T>* v) {
   const T c1 = T(k::COS_PI16), s1 = T(k::SIN_PI16);
   const T c2 = T(k::COS_PI8), s2 = T(k::SIN_PI8);
   const T c3 = T(k::COS_3PI16), s3 = T(k::SIN_3PI16);
@@ -175,7 +175,7 @@ template <typename T> __host__ __device__ inline void bfly32(V2<T>* v) {
   }
 }
 
-template <typename T, int R> __host__ __device__ inline void bfly(V2<T>* v) {
+template <typename T, int R> __host__ __device__ __forceinline__ void bfly(V2<T>* v) {
   if constexpr (R == 2) bfly2<T>(v);
   else if constexpr (R == 3) bfly3<T>(v);
   else if constexpr (R == 4) bfly4<T>(v);

@@ -40,17 +40,44 @@ int hip_fail(hipError_t e, const char *what) {
     if (e_ != hipSuccess) return hip_fail(e_, what); \
   } while (0)
 
+// The kernel registry: every table's entries, with the grouped and segmented
+// launchers (compiled in units of their own, listed in the same order)
+// merged in by index.
 struct Registry {
+  std::vector<KernelEntry> storage;
   std::vector<const KernelEntry *> entries;
   Registry() {
-    const KernelEntry *(*fns[])(int *) = {table_f32_row,  table_f32_col, table_f32_real,
-                                          table_f64_row,  table_f64_col, table_f64_real,
-                                          table_split};
-    for (auto fn : fns) {
-      int n = 0;
-      const KernelEntry *t = fn(&n);
-      for (int i = 0; i < n; ++i) entries.push_back(&t[i]);
+    struct Src {
+      const KernelEntry *(*table)(int *);
+      const GroupLaunchFn *(*group)(int *);
+      const LaunchFn *(*seg)(int *);
+    };
+    const Src srcs[] = {{table_f32_row, group_f32_row, seg_f32_row},
+                        {table_f32_col, group_f32_col, seg_f32_col},
+                        {table_f32_real, group_f32_real, seg_f32_real},
+                        {table_f64_row, group_f64_row, seg_f64_row},
+                        {table_f64_col, group_f64_col, seg_f64_col},
+                        {table_f64_real, group_f64_real, seg_f64_real},
+                        {table_f32_w3232, group_f32_w3232, seg_f32_w3232},
+                        {table_split, nullptr, nullptr}};
+    for (const Src &src : srcs) {
+      int n = 0, ng = 0, ns = 0;
+      const KernelEntry *t = src.table(&n);
+      const GroupLaunchFn *g = src.group ? src.group(&ng) : nullptr;
+      const LaunchFn *sg = src.seg ? src.seg(&ns) : nullptr;
+      if ((g && ng != n) || (sg && ns != n)) {
+        fprintf(stderr, "pfx: kernel lists out of step (%d entries, %d grouped, %d segmented)\n",
+                n, ng, ns);
+        abort();
+      }
+      for (int i = 0; i < n; ++i) {
+        KernelEntry e = t[i];
+        e.launch_group = g ? g[i] : nullptr;
+        e.launch_seg = sg ? sg[i] : nullptr;
+        storage.push_back(e);
+      }
     }
+    for (const KernelEntry &e : storage) entries.push_back(&e);
   }
 };
 
@@ -419,7 +446,7 @@ int pfx_execute_segmented(pfx_plan p, int direction, void *data, const pfx_segma
                           const pfx_segmap *out, double scale, pfx_stream stream) {
   if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
   if (!in && !out) return pfx_execute_scaled(p, direction, data, scale, stream);
-  if (!p->kernel || p->kernel->prec > 1)
+  if (!p->kernel || p->kernel->prec > 1 || !p->kernel->launch_seg)
     return fail(PFX_ERR_UNSUPPORTED, "segmented execution needs a tiled plain-precision kernel");
   if (p->kind == PFX_C2R || (p->kind == PFX_R2C && in))
     return fail(PFX_ERR_UNSUPPORTED, "segmented loads are for C2C plans; C2R is not fused");
@@ -467,7 +494,8 @@ int pfx_execute_segmented(pfx_plan p, int direction, void *data, const pfx_segma
     int rc = to_segmap(out, real ? (int)p->length / 2 + 1 : (int)p->length, total, &a.out_map);
     if (rc) return rc;
   }
-  p->kernel->launch(a, static_cast<hipStream_t>(stream));
+  (a.in_seg || a.out_seg ? p->kernel->launch_seg : p->kernel->launch)(
+      a, static_cast<hipStream_t>(stream));
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "kernel launch");
   return PFX_SUCCESS;
@@ -968,9 +996,61 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
     int rc = pfx_plan_tune(s, flag, scratch, stream);
     if (rc != PFX_SUCCESS) return rc;
   }
-  for (pfx_plan s : p->cstages) {
-    int rc = pfx_plan_tune(s, flag, scratch, stream);
-    if (rc != PFX_SUCCESS) return rc;
+  // Chunk plans start from the kernel and tile order the full stage was
+  // tuned to; then each chunked forward stage is re-tuned in context, timing
+  // the whole chunked schedule per candidate (a chunk's short launches alone
+  // are too noisy, and the best kernel for a pass that reads from the
+  // Infinity Cache differs from the best for one that reads HBM).
+  for (size_t c = 0; c < p->cstages.size(); ++c) {
+    const size_t src = (p->kind == PFX_R2C && c + 1 == p->cstages.size()) ? p->ndim : c;
+    pfx_plan full = p->stages[src], part = p->cstages[c];
+    if (flag != PFX_PLAN_ESTIMATE && full->kernel && part->kernel) {
+      PFX_HIP(set_kernel(part, full->kernel), "chunk kernel");
+      part->xcd_remap = full->xcd_remap;
+    }
+  }
+  if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE) {
+    hipStream_t s = static_cast<hipStream_t>(stream);
+    hipEvent_t a, b;
+    PFX_HIP(hipEventCreate(&a), "hipEventCreate");
+    PFX_HIP(hipEventCreate(&b), "hipEventCreate");
+    const int reps = flag == PFX_PLAN_PATIENT ? 4 : 2;
+    auto time_schedule = [&](float *ms) -> int {
+      int rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);  // warm-up
+      hipEventRecord(a, s);
+      for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r)
+        rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);
+      hipEventRecord(b, s);
+      if (hipEventSynchronize(b) != hipSuccess) return PFX_ERR_HIP;
+      hipEventElapsedTime(ms, a, b);
+      return rc;
+    };
+    for (int st = 0; st + 1 < p->ndim; ++st) {
+      pfx_plan part = p->cstages[st];
+      if (!part->kernel) continue;
+      const KernelEntry *best = part->kernel;
+      int best_order = part->xcd_remap;
+      float best_ms = -1.f;
+      for (const KernelEntry *k : candidates(part)) {
+        if (set_kernel(part, k) != hipSuccess) continue;
+        std::vector<int> orders = {part->xcd_remap};
+        if (k->col) orders = {1, 8, 32};
+        for (int order : orders) {
+          part->xcd_remap = order;
+          float ms = 0.f;
+          if (time_schedule(&ms) != PFX_SUCCESS) continue;
+          if (best_ms < 0.f || ms < best_ms) {
+            best_ms = ms;
+            best = k;
+            best_order = order;
+          }
+        }
+      }
+      PFX_HIP(set_kernel(part, best), "chunk kernel");
+      part->xcd_remap = best_order;
+    }
+    hipEventDestroy(a);
+    hipEventDestroy(b);
   }
   if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE) {
     // keep the chunked schedule only if it times faster than stage-by-stage

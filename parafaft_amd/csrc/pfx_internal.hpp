@@ -29,49 +29,30 @@ struct KernelEntry {
   LaunchFn launch;
   Radices rad;    // pass radices, first pass first
   GroupLaunchFn launch_group;  // null: no grouped form
+  LaunchFn launch_seg;         // segmented form (fused exchange); null: none
 };
 
 template <int... Rs> constexpr int prod() { return (Rs * ... * 1); }
 
+// Plain launchers (in place, or out of place through StockhamArgs::src).
+// The grouped and segmented launchers live in pfx_launch_extra.hpp and are
+// compiled in units of their own: a unit that holds two kernels built from
+// the same body (plain and grouped, or plain and segmented) was measured to
+// allocate registers differently and spill in the plain kernel.
 template <typename T, int TPT, int C, bool COL, int... Rs>
 void launch_c2c(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
-  if (a.in_seg || a.out_seg)
-    c2c_kernel<T, TPT, C, COL, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
-  else
-    c2c_kernel<T, TPT, C, COL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  c2c_kernel<T, TPT, C, COL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, bool ROWL, int... Rs>
 void launch_c2c_wide(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
-  if (a.in_seg || a.out_seg)
-    c2c_wide_kernel<T, TPT, C, ROWL, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
-  else
-    c2c_wide_kernel<T, TPT, C, ROWL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
-}
-template <typename T, int TPT, int C, bool COL, int... Rs>
-void group_c2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
-  c2c_group_kernel<T, TPT, C, COL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
-}
-template <typename T, int TPT, int C, bool ROWL, int... Rs>
-void group_c2c_wide(const GroupArgs &g, int64_t blocks, hipStream_t s) {
-  c2c_wide_group_kernel<T, TPT, C, ROWL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
-}
-template <typename T, int TPT, int C, int... Rs>
-void group_r2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
-  r2c_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
-}
-template <typename T, int TPT, int C, int... Rs>
-void group_c2r(const GroupArgs &g, int64_t blocks, hipStream_t s) {
-  c2r_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
+  c2c_wide_kernel<T, TPT, C, ROWL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, int... Rs>
 void launch_r2c(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
-  if (a.out_seg)
-    r2c_kernel<T, TPT, C, true, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
-  else
-    r2c_kernel<T, TPT, C, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+  r2c_kernel<T, TPT, C, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
 template <typename T, int TPT, int C, int... Rs>
 void launch_c2r(const StockhamArgs &a, hipStream_t s) {
@@ -83,7 +64,9 @@ template <typename T> constexpr int prec_of();
 template <> constexpr int prec_of<float>() { return 0; }
 template <> constexpr int prec_of<double>() { return 1; }
 
-// Registry tables, one per instantiation unit.
+// Registry tables, one per instantiation unit.  The grouped and segmented
+// launcher lists run parallel to the entry list of the same name (same
+// order and count); the registry merges them into the entries.
 const KernelEntry *table_f32_row(int *count);
 const KernelEntry *table_f32_col(int *count);
 const KernelEntry *table_f32_real(int *count);
@@ -91,6 +74,21 @@ const KernelEntry *table_f64_row(int *count);
 const KernelEntry *table_f64_col(int *count);
 const KernelEntry *table_f64_real(int *count);
 const KernelEntry *table_split(int *count);
+const KernelEntry *table_f32_w3232(int *count);
+const GroupLaunchFn *group_f32_w3232(int *count);
+const LaunchFn *seg_f32_w3232(int *count);
+const GroupLaunchFn *group_f32_row(int *count);
+const GroupLaunchFn *group_f32_col(int *count);
+const GroupLaunchFn *group_f32_real(int *count);
+const GroupLaunchFn *group_f64_row(int *count);
+const GroupLaunchFn *group_f64_col(int *count);
+const GroupLaunchFn *group_f64_real(int *count);
+const LaunchFn *seg_f32_row(int *count);
+const LaunchFn *seg_f32_col(int *count);
+const LaunchFn *seg_f32_real(int *count);
+const LaunchFn *seg_f64_row(int *count);
+const LaunchFn *seg_f64_col(int *count);
+const LaunchFn *seg_f64_real(int *count);
 
 // Fallback kernels (any length that fits LDS): direct O(n^2) DFT.
 struct NaiveArgs {

@@ -1,0 +1,5 @@
+#define PFX_T float
+#define PFX_T_IS_DOUBLE 0
+#define PFX_MODE 2
+#define PFX_TABLE_FN seg_f32_row
+#include "pfx_tab_row.inc"

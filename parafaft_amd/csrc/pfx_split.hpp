@@ -33,23 +33,23 @@ typedef short bf16x4_t __attribute__((ext_vector_type(4)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // float -> bf16 bits, round to nearest even (finite inputs)
-__device__ inline short bf16_rne(float f) {
+__device__ __forceinline__ short bf16_rne(float f) {
   uint32_t u = __float_as_uint(f);
   u += 0x7FFFu + ((u >> 16) & 1u);
   return (short)(u >> 16);
 }
-__device__ inline float bf16_to_f(short h) {
+__device__ __forceinline__ float bf16_to_f(short h) {
   return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
 }
 
-__device__ inline float wave_max(float v) {
+__device__ __forceinline__ float wave_max(float v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
 }
 
 // cos / sin of 2 pi t / 16, t = 0..15
-__device__ inline void root16(int t, float& c, float& s) {
+__device__ __forceinline__ void root16(int t, float& c, float& s) {
   const float a = float(k::COS_PI8), b = float(k::SIN_PI8), r = float(k::SQRT1_2);
   const float cs[16] = {1.f, a, r, b, 0.f, -b, -r, -a, -1.f, -a, -r, -b, 0.f, b, r, a};
   c = cs[t & 15];
@@ -118,7 +118,7 @@ template <> struct MfmaDft16<SPLIT_F16> {
 
 template <> struct MfmaDft16<SPLIT_BF16> {
   bf16x4_t f[3][3];  // [matrix: Fr, Fi, -Fi][term: hi, mid, lo]
-  __device__ static inline void split3(float v, short& h, short& m, short& l) {
+  __device__ static __forceinline__ void split3(float v, short& h, short& m, short& l) {
     h = bf16_rne(v);
     const float r1 = v - bf16_to_f(h);
     m = bf16_rne(r1);
@@ -141,7 +141,7 @@ template <> struct MfmaDft16<SPLIT_BF16> {
       }
     }
   }
-  __device__ static inline float4_t prod(const bf16x4_t* a, const bf16x4_t* b, float4_t z) {
+  __device__ static __forceinline__ float4_t prod(const bf16x4_t* a, const bf16x4_t* b, float4_t z) {
     // terms of order <= 2 in the split: lo*hi, mid*mid, hi*lo, then mid*hi,
     // hi*mid, then hi*hi
     z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[2], b[0], z, 0, 0, 0);
@@ -179,7 +179,7 @@ template <> struct MfmaDft16<SPLIT_BF16> {
 // One radix-16 Stockham pass over the workgroup's LDS tile on the matrix
 // cores: groups of 16 butterflies of one transform per MFMA product.
 template <int MODE, int N, int TPT, int C, bool COL, int Ns>
-__device__ inline void mfma_pass16(V2<float>* lds, const V2<float>* stw, const MfmaDft16<MODE>& dft) {
+__device__ __forceinline__ void mfma_pass16(V2<float>* lds, const V2<float>* stw, const MfmaDft16<MODE>& dft) {
   using L = LdsLayout<float, N, C, COL>;
   constexpr int NB = N / 16;     // butterflies per transform
   constexpr int GPT = NB / 16;   // groups per transform
@@ -223,13 +223,13 @@ struct SplitPasses;
 
 template <int MODE, int N, int TPT, int C, bool COL, int Ns>
 struct SplitPasses<MODE, N, TPT, C, COL, Ns> {
-  __device__ static inline void run(int, int, V2<float>*, const V2<float>*, const MfmaDft16<MODE>&) {}
+  __device__ static __forceinline__ void run(int, int, V2<float>*, const V2<float>*, const MfmaDft16<MODE>&) {}
 };
 
 template <int MODE, int N, int TPT, int C, bool COL, int Ns, int R, int... Rest>
 struct SplitPasses<MODE, N, TPT, C, COL, Ns, R, Rest...> {
   using L = LdsLayout<float, N, C, COL>;
-  __device__ static inline void run(int j, int c, V2<float>* lds, const V2<float>* stw,
+  __device__ static __forceinline__ void run(int j, int c, V2<float>* lds, const V2<float>* stw,
                                     const MfmaDft16<MODE>& dft) {
     if constexpr (R == 16) {
       mfma_pass16<MODE, N, TPT, C, COL, Ns>(lds, stw, dft);
@@ -253,8 +253,10 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
   constexpr int N = (Rs * ... * 1);
   constexpr int E = N / TPT;
   using L = LdsLayout<float, N, C, COL>;
-  __shared__ V2<float> lds[L::size];
-  __shared__ V2<float> stw[N];
+  using B = LdsBlock<float, V2<float>, N, L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<float>* lds = B::data(smem);
+  V2<float>* stw = B::tw(smem);
   const V2<float>* tw = stage_twiddles<float, N, sizeof...(Rs)>(stw, a.ptw);
   const MfmaDft16<MODE> dft;
   int j, c;

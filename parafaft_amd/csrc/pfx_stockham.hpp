@@ -104,7 +104,7 @@ struct StockhamArgs {
 // a cache line must run on the same XCD, so tile ids are remapped to give XCD
 // x the contiguous range [x q + min(x, r), ...) of the nb = 8 q + r tiles.
 constexpr int kNumXcd = 8;
-__device__ inline int64_t xcd_tile(int64_t i, int64_t nb) {
+__device__ __forceinline__ int64_t xcd_tile(int64_t i, int64_t nb) {
   const int64_t q = nb / kNumXcd, r = nb % kNumXcd;
   const int64_t x = i % kNumXcd, k = i / kNumXcd;
   return x * q + (x < r ? x : r) + k;
@@ -112,12 +112,12 @@ __device__ inline int64_t xcd_tile(int64_t i, int64_t nb) {
 // XCD x takes chunks x, x + 8, x + 16, ... of K consecutive tiles, so all
 // XCDs work inside one window of 8 K tiles at a time.  Used when 8 K divides
 // the tile count; otherwise the contiguous-run order.
-__device__ inline int64_t xcd_tile_chunked(int64_t i, int64_t nb, int64_t K) {
+__device__ __forceinline__ int64_t xcd_tile_chunked(int64_t i, int64_t nb, int64_t K) {
   if (K <= 1 || nb % (kNumXcd * K) != 0) return xcd_tile(i, nb);
   const int64_t x = i % kNumXcd, k = i / kNumXcd;
   return ((k / K) * kNumXcd + x) * K + (k % K);
 }
-__device__ inline int64_t tile_order(int64_t i, int64_t nb, int mode) {
+__device__ __forceinline__ int64_t tile_order(int64_t i, int64_t nb, int mode) {
   if (mode == 0) return i;
   if (mode == 1) return xcd_tile(i, nb);
   return xcd_tile_chunked(i, nb, mode);
@@ -128,7 +128,7 @@ template <typename T, int N, int C, bool COL> struct LdsLayout {
   // keeps the first pass's stride-R writes off a single bank.
   static constexpr int NP = N + N / 16 + 1;
   static constexpr int size = COL ? N * C : NP * C;
-  __device__ static inline int at(int idx, int c) {
+  __device__ static __forceinline__ int at(int idx, int c) {
     if constexpr (COL) return idx * C + c;
     else return c * NP + idx + (idx >> 4);
   }
@@ -137,7 +137,7 @@ template <typename T, int N, int C, bool COL> struct LdsLayout {
 // One Stockham pass executed by one thread for its Q butterflies.
 // SRC/DST: 0 = LDS, 1 = HBM.
 template <typename T, int N, int R, int Ns, int TPT, class LoadF, class StoreF>
-__device__ inline void stockham_pass(int j, const V2<T>* __restrict__ stw, LoadF load, StoreF store) {
+__device__ __forceinline__ void stockham_pass(int j, const V2<T>* __restrict__ stw, LoadF load, StoreF store) {
   constexpr int NB = N / R;                  // butterflies in this pass
   constexpr int Q = (NB + TPT - 1) / TPT;    // butterflies per thread
   constexpr bool EXACT = (NB % TPT) == 0;
@@ -187,8 +187,8 @@ template <typename T, bool SEG = false> struct GlobalPtr {
   bool valid;
   T scale;
   bool use_scale;
-  __device__ inline V2<T> load(int e) const { return pin[(int64_t)e * stride]; }
-  __device__ inline void store(int e, V2<T> v) const { p[(int64_t)e * stride] = v; }
+  __device__ __forceinline__ V2<T> load(int e) const { return pin[(int64_t)e * stride]; }
+  __device__ __forceinline__ void store(int e, V2<T> v) const { p[(int64_t)e * stride] = v; }
 };
 
 template <typename T> struct GlobalPtr<T, true> {
@@ -203,24 +203,24 @@ template <typename T> struct GlobalPtr<T, true> {
   KSegMap* out_map;          // non-null: stores go to a segment layout
   uint32_t in_lead, out_lead;  // transform's place in each map
   int64_t in_trail, out_trail;
-  __device__ inline V2<T> load(int e) const {
+  __device__ __forceinline__ V2<T> load(int e) const {
     if (in_map) return *seg_addr<T>(in_map, in_lead, in_trail, e);
     return pin[(int64_t)e * stride];
   }
-  __device__ inline void store(int e, V2<T> v) const {
+  __device__ __forceinline__ void store(int e, V2<T> v) const {
     if (out_map) *seg_addr<T>(out_map, out_lead, out_trail, e) = v;
     else p[(int64_t)e * stride] = v;
   }
 };
 
-__device__ inline int64_t xform_offset(int64_t b, const StockhamArgs& a) {
+__device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a) {
   const int64_t o = b / a.inner;
   const int64_t i = b - o * a.inner;
   return o * a.dist_outer + i * a.dist;
 }
 
 template <typename T, bool SEG>
-__device__ inline void init_global(GlobalPtr<T, SEG>& g, const StockhamArgs& a, int64_t b) {
+__device__ __forceinline__ void init_global(GlobalPtr<T, SEG>& g, const StockhamArgs& a, int64_t b) {
   g.valid = b < a.total;
   const V2<T>* in_base = static_cast<const V2<T>*>(a.src ? a.src : a.data);
   if constexpr (!SEG) {
@@ -261,7 +261,7 @@ struct Passes;
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
           int Ns>
 struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns> {
-  template <class G> __device__ static inline void run(int, int, V2<T>*, const V2<T>*, const G&) {}
+  template <class G> __device__ static __forceinline__ void run(int, int, V2<T>*, const V2<T>*, const G&) {}
 };
 
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
@@ -269,7 +269,7 @@ template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bo
 struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
   using L = LdsLayout<T, N, C, COL>;
   template <class G>
-  __device__ static inline void run(int j, int c, V2<T>* lds, const V2<T>* tw, const G& g) {
+  __device__ static __forceinline__ void run(int j, int c, V2<T>* lds, const V2<T>* tw, const G& g) {
     constexpr bool IS_FIRST = (PI == 0);
     constexpr bool IS_LAST = (sizeof...(Rest) == 0);
     auto load = [&](int e) -> V2<T> {
@@ -313,7 +313,7 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
 // Copy the per-pass twiddle table (N - 1 entries) into LDS; only plans with
 // more than one pass multiply by twiddles.
 template <typename T, int N, int NPASS>
-__device__ inline const V2<T>* stage_twiddles(V2<T>* stw, const void* ptw) {
+__device__ __forceinline__ const V2<T>* stage_twiddles(V2<T>* stw, const void* ptw) {
   if constexpr (NPASS > 1) {
     const V2<T>* src = static_cast<const V2<T>*>(ptw);
     for (int i = threadIdx.x; i < N - 1; i += blockDim.x) stw[i] = src[i];
@@ -322,8 +322,25 @@ __device__ inline const V2<T>* stage_twiddles(V2<T>* stw, const void* ptw) {
   return stw;
 }
 
+// One LDS block per kernel body, the per-pass twiddle table (TW entries of
+// V2<T>) first and the data tile (DATA entries of D) after it.  A single
+// block fixes the layout: separately declared arrays are placed by the
+// compiler, and with the small twiddle table behind a 128 KB tile its
+// addresses no longer fit the 16-bit offset of ds_read, which costs one
+// VGPR per twiddle address and spilled the 1024-thread kernels.
+template <typename T, typename D, int TW, int DATA> struct LdsBlock {
+  static constexpr int tw_bytes = (TW * (int)sizeof(V2<T>) + 15) / 16 * 16;
+  static constexpr int bytes = tw_bytes + DATA * (int)sizeof(D);
+  __device__ static __forceinline__ V2<T>* tw(unsigned char* b) {
+    return reinterpret_cast<V2<T>*>(b);
+  }
+  __device__ static __forceinline__ D* data(unsigned char* b) {
+    return reinterpret_cast<D*>(b + tw_bytes);
+  }
+};
+
 template <int TPT, int C, bool COL>
-__device__ inline void thread_coords(int& j, int& c) {
+__device__ __forceinline__ void thread_coords(int& j, int& c) {
   const int tid = threadIdx.x;
   if constexpr (COL) { c = tid % C; j = tid / C; }
   else { j = tid % TPT; c = tid / TPT; }
@@ -336,11 +353,13 @@ __device__ inline void thread_coords(int& j, int& c) {
 // launch (or of its job in a grouped launch), so the single-plan kernels and
 // the grouped multi-transform kernels share them.
 template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
-__device__ inline void c2c_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+__device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, COL>;
-  __shared__ V2<T> lds[L::size];
-  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<T>* lds = B::data(smem);
+  V2<T>* stw = B::tw(smem);
   const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
@@ -367,11 +386,13 @@ __global__ void __launch_bounds__(TPT * C) c2c_kernel(StockhamArgs a) {
 // Row layout only (stride 1).  a.dist is in complex units (= padded dist / 2).
 // ---------------------------------------------------------------------------
 template <typename T, int TPT, int C, bool SEG, int... Rs>
-__device__ inline void r2c_body(const StockhamArgs& a, int64_t tile) {
+__device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
-  __shared__ V2<T> lds[L::size];
-  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<T>* lds = B::data(smem);
+  V2<T>* stw = B::tw(smem);
   const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
@@ -423,11 +444,13 @@ __global__ void __launch_bounds__(TPT * C) r2c_kernel(StockhamArgs a) {
 // gives z[n] = x[2n] + i x[2n+1], stored as the N complex slots of the row.
 // ---------------------------------------------------------------------------
 template <typename T, int TPT, int C, bool SEG, int... Rs>
-__device__ inline void c2r_body(const StockhamArgs& a, int64_t tile) {
+__device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
-  __shared__ V2<T> lds[L::size];
-  __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
+  using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<T>* lds = B::data(smem);
+  V2<T>* stw = B::tw(smem);
   const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
@@ -492,7 +515,7 @@ template <int N, int C, bool ROWL> struct WideLayout {
   // with Ns = 1 off a single bank
   static constexpr int NP = N + N / 16 + 1;
   static constexpr int size = ROWL ? NP * C : N * C;
-  __device__ static inline int at(int e, int c) {
+  __device__ static __forceinline__ int at(int e, int c) {
     if constexpr (ROWL) return c * NP + e + (e >> 4);
     else return e * C + c;
   }
@@ -581,7 +604,9 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
   constexpr int Q0 = E / R0;
   constexpr int NB0 = N / R0;
   using WL = WideLayout<N, C, ROWL>;
-  __shared__ T lds[WL::size];
+  using B = LdsBlock<T, T, (!ROWL && sizeof...(Rs) > 1 ? N : 0), WL::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  T* lds = B::data(smem);
   const V2<T>* tw;
   int c, j;
   if constexpr (ROWL) {
@@ -589,8 +614,7 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
     j = threadIdx.x % TPT;
     c = threadIdx.x / TPT;
   } else {
-    __shared__ V2<T> stw[sizeof...(Rs) > 1 ? N : 1];
-    tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
+    tw = stage_twiddles<T, N, sizeof...(Rs)>(B::tw(smem), a.ptw);
     c = threadIdx.x % C;
     j = threadIdx.x / C;
     tile = tile_order(tile, ntiles, a.xcd_remap);
@@ -633,7 +657,7 @@ struct GroupArgs {
   int njobs;
 };
 
-__device__ inline int find_job(const GroupArgs& ga, int64_t blk) {
+__device__ __forceinline__ int find_job(const GroupArgs& ga, int64_t blk) {
   int lo = 0, hi = ga.njobs - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;

@@ -1,4 +1,5 @@
 #define PFX_T double
 #define PFX_T_IS_DOUBLE 1
+#define PFX_MODE 0
 #define PFX_TABLE_FN table_f64_real
 #include "pfx_tab_real.inc"
