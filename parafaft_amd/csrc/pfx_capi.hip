@@ -1052,7 +1052,11 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
     hipEventDestroy(a);
     hipEventDestroy(b);
   }
-  if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE) {
+  // PFX_ND_SCHEDULE=chunked|stagewise overrides the comparison below (for
+  // measurement, e.g. counter runs that serialise dispatches)
+  const char *forced = getenv("PFX_ND_SCHEDULE");
+  if (forced && !strcmp(forced, "stagewise")) return pfx_ndplan_set_chunk(p, 0);
+  if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE && !(forced && !strcmp(forced, "chunked"))) {
     // keep the chunked schedule only if it times faster than stage-by-stage
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t a, b;
