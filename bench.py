@@ -69,6 +69,13 @@ def bench_pencil(args, torch, A):
                            fused=not args.unfused)
     rdt = torch.float64 if share.esz == 16 else torch.float32
     buf = torch.zeros(share.get_required_output_size(), dtype=rdt, device="cuda")
+    kernels = None
+    if args.plan != "estimate":
+        t0 = time.perf_counter()
+        kernels = share.tune(buf)     # Measure planning on the share's own buffer
+        log(f"[bench] pencil planning: {time.perf_counter() - t0:.1f} s, kernels {kernels}")
+        buf.zero_()
+        torch.cuda.synchronize()
     g = torch.Generator(device="cuda")
     g.manual_seed(3)
     real = torch.randn((512 * 512, 2048), generator=g, device="cuda", dtype=rdt)
@@ -123,7 +130,8 @@ def bench_pencil(args, torch, A):
     print(json.dumps({
         "workload": "R2C pencil share: rank (0,0) of 2048^3 on a 4x4 grid, 512x512x2048 real, "
                     + how, "fused": share.fused,
-        "precision": args.precision, "steps": args.steps, "ms_per_forward": round(ms, 4),
+        "precision": args.precision, "planning": args.plan, "stage_kernels": kernels,
+        "steps": args.steps, "ms_per_forward": round(ms, 4),
         "part_ms": {k: round(v, 4) for k, v in parts.items()},
         "part_tbs": {k: round(alg[k] / (parts[k] * 1e-3) / 1e12, 3) for k in parts},
         "algorithmic_bytes": total, "tbs_algorithmic": round(total / (ms * 1e-3) / 1e12, 3),

@@ -405,6 +405,12 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   g.use_scale = a.use_scale != 0;
   const V2<T>* tw = static_cast<const V2<T>*>(a.tw);  // W_{2N}^i, i < 2N
   Passes<T, N, TPT, C, false, 2, true, false, 0, 1, Rs...>::run(j, c, lds, ptw, g);
+  // Segmented output: the post-twiddle writes X back into the LDS row (each
+  // thread owns its pairs (k, N-k), so in place is safe; X[N] takes the
+  // row's spare slot N), then one uniform loop over the peers' blocks stores
+  // each block as a run, with no per-element peer search.
+  bool to_lds = false;
+  if constexpr (SEG) to_lds = g.out_map != nullptr;
   // post-twiddle: pairs (k, N-k) for k = 0..N/2
   const T half = T(0.5);
   for (int k = j; k <= N / 2; k += TPT) {
@@ -418,7 +424,8 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
       V2<T> wd = cmul<T>(tw[k], d);
       V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
       if (g.use_scale) x = scl<T>(x, g.scale);
-      if (g.valid) g.store(k, x);
+      if (to_lds) lds[L::at(k, c)] = x;
+      else if (g.valid) g.store(k, x);
     }
     // X[N-k] unless k is its own mirror (2k == N); k = 0 gives X[N]
     if (2 * k != N) {
@@ -428,7 +435,22 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
       V2<T> wd = cmul<T>(tw[kk], d);
       V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
       if (g.use_scale) x = scl<T>(x, g.scale);
-      if (g.valid) g.store(kk, x);
+      if (to_lds) lds[L::at(kk, c)] = x;
+      else if (g.valid) g.store(kk, x);
+    }
+  }
+  if constexpr (SEG) {
+    if (to_lds) {
+      __syncthreads();
+      KSegMap* m = g.out_map;
+      const int64_t T_ = m->trailing;
+      for (int q = 0; q < m->nparts; ++q) {
+        const int s0 = m->s[q], n0 = m->n[q];
+        V2<T>* base = static_cast<V2<T>*>(m->seg[q]) + g.out_trail;
+        const uint32_t row0 = g.out_lead * (uint32_t)n0;
+        for (int e = s0 + j; e < s0 + n0; e += TPT)
+          if (g.valid) base[(int64_t)(row0 + (uint32_t)(e - s0)) * T_] = lds[L::at(e, c)];
+      }
     }
   }
 }

@@ -150,6 +150,41 @@ class PencilShareR2C:
                                     g.dst_trailing, g.dst_axis_extent, self.esz, self.stream),
                 "pfx_unpack")
 
+    def tune(self, padded, reps: int = 2):
+        """Measure planning for this share's schedule (the FFTW_MEASURE analogue
+        for the whole forward, fused or stepwise): for each stage plan in turn,
+        every candidate kernel is timed inside the full forward on `padded`
+        (contents destroyed) and the fastest is kept.  Kernels that read or
+        write segments carry extra addressing registers, so the best choice
+        for a fused stage can differ from the best for the plain stage."""
+        import time
+        L = A.load()
+
+        def timed():
+            self.forward_in_place(padded)
+            A.check(L.pfx_stream_synchronize(self.stream), "sync")
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                self.forward_in_place(padded)
+            A.check(L.pfx_stream_synchronize(self.stream), "sync")
+            return time.perf_counter() - t0
+
+        chosen = []
+        for plan in [self.r2c] + self.plans[1:]:
+            best_i, best_t = None, None
+            for i in range(plan.num_candidates()):
+                try:
+                    plan.use_candidate(i)
+                    t = timed()
+                except A.PfxError:
+                    continue
+                if best_t is None or t < best_t:
+                    best_i, best_t = i, t
+            if best_i is not None:
+                plan.use_candidate(best_i)
+            chosen.append(plan.describe()[0])
+        return chosen
+
     def segmap(self, t, side):
         """pfx_segmap of transition t: side "out" is where stage t stores
         (send segments, the rank's own block in its receive segment), side

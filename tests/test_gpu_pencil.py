@@ -289,3 +289,21 @@ def test_fused_rejects_bad_maps():
     rc = A.load().pfx_execute_segmented(plan.handle, A.PFX_FORWARD, d.data_ptr(), C.byref(m),
                                         None, 1.0, None)
     assert rc == A.PFX_ERR_INVALID_ARG
+
+
+@pytest.mark.parametrize("fused", [True, False])
+def test_share_tuned_vs_oracle(fused):
+    """Measure planning of the share (every candidate kernel timed inside the
+    whole forward), then the forward against the FP64 oracle pipeline."""
+    from parafaft_amd.pencil import PencilShareR2C
+    G = [128, 128, 128]
+    share = PencilShareR2C(G, [4, 4], [1, 2], precision="f32", fused=fused)
+    scratch = to_dev(np.zeros(share.get_required_output_size(), np.float32))
+    kernels = share.tune(scratch)
+    assert len(kernels) == 3
+    real = filtered_noise(tuple(share.stage_shapes[0]), seed=77).astype(np.float32)
+    peers = _fill_peers(share, 640, np.complex64)
+    got = _run_fused(share, real, np.float32) if fused else \
+        _run_stepwise(share, real, np.float32, check_bytes=False).view(np.uint8).ravel()
+    ref = _oracle_share(real.astype(np.float64), share, peers)
+    assert rel_l2(got.view(np.complex64).reshape(ref.shape), ref) <= 1e-6 * 21
