@@ -324,7 +324,9 @@ def main():
     stream = torch.cuda.current_stream()
     sp = stream.cuda_stream
 
-    log(f"[bench] allocating {N * 8 / 2**30:.1f} GiB complex<float> {n}^3")
+    esz_c = 16 if prec == A.PFX_F64 else 8
+    log(f"[bench] allocating {N * esz_c / 2**30:.1f} GiB "
+        f"complex<{'double' if prec == A.PFX_F64 else 'float'}> {n}^3")
     g = torch.Generator(device=dev)
     g.manual_seed(20241203)
     rdt = torch.float64 if prec == A.PFX_F64 else torch.float32
@@ -393,7 +395,8 @@ def main():
     # timed parts: the D stages, or (chunked) axes 2+1 together and axis 0
     parts = [[h] for h in stages] if not chunk else [None, [stages[D - 1]]]
 
-    def step(events=None):
+    def step(events=None, st=None):
+        st = st or sp
         base = x.data_ptr()
         for s, hs in enumerate(parts):
             if events is not None:
@@ -401,12 +404,12 @@ def main():
             if hs is None:
                 for c in range(n // chunk):
                     for h in cstages:
-                        rc = L.pfx_execute(h, A.PFX_FORWARD, base + c * chunk_bytes, sp)
+                        rc = L.pfx_execute(h, A.PFX_FORWARD, base + c * chunk_bytes, st)
                         if rc:
                             A.check(rc, "pfx_execute")
             else:
                 for h in hs:
-                    rc = L.pfx_execute(h, A.PFX_FORWARD, base, sp)
+                    rc = L.pfx_execute(h, A.PFX_FORWARD, base, st)
                     if rc:
                         A.check(rc, "pfx_execute")
             if events is not None:
@@ -433,6 +436,31 @@ def main():
                for s in range(NP)]
     log(f"[bench] ms/step {ms_per_step:.3f}, part ms {['%.3f' % t for t in part_ms]}")
 
+    # the same forward captured once as a HIP graph and replayed (outside the
+    # timed region above; reported beside it)
+    graph_ms = None
+    try:
+        gstream = torch.cuda.Stream()      # the legacy null stream cannot be captured
+        gsp = gstream.cuda_stream
+        gh = C.c_void_p()
+        torch.cuda.synchronize()
+        A.check(L.pfx_graph_begin_capture(gsp), "pfx_graph_begin_capture")
+        try:
+            step(None, gsp)
+        finally:
+            A.check(L.pfx_graph_end_capture(gsp, C.byref(gh)), "pfx_graph_end_capture")
+        A.check(L.pfx_graph_launch(gh.value, gsp), "pfx_graph_launch")
+        torch.cuda.synchronize()
+        g0 = time.perf_counter()
+        for _ in range(args.steps):
+            A.check(L.pfx_graph_launch(gh.value, gsp), "pfx_graph_launch")
+        torch.cuda.synchronize()
+        graph_ms = (time.perf_counter() - g0) * 1e3 / args.steps
+        L.pfx_graph_destroy(gh.value)
+        log(f"[bench] graph replay ms/step {graph_ms:.3f}")
+    except Exception as e:  # graphs are reported, not required
+        log(f"[bench] graph replay unavailable: {e}")
+
     flops = 5.0 * N * math.log2(N)
     gflops = flops / (ms_per_step * 1e-3) / 1e9
     alg_bytes = 2.0 * D * N * esz
@@ -449,7 +477,8 @@ def main():
 
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_1024cube_f32.json")
-    if os.path.exists(tpath):
+    # the committed counter profile covers the 1024^3 FP32 configuration only
+    if os.path.exists(tpath) and n == 1024 and prec == A.PFX_F32:
         try:
             with open(tpath) as f:
                 t = json.load(f)
@@ -479,7 +508,8 @@ def main():
         "vs_baseline": None,
         "dtype": {A.PFX_F32: "fp32", A.PFX_F64: "fp64"}.get(prec, args.precision),
         "data": "synthetic: iid Uniform[-1,1) re/im (seeded), resident in HBM",
-        "config": {"workload": f"3D C2C forward, in place, {n}^3 complex<float>, one GPU "
+        "config": {"workload": f"3D C2C forward, in place, {n}^3 "
+                               f"complex<{'double' if prec == A.PFX_F64 else 'float'}>, one GPU "
                                "(one-rank fast path: "
                                + (f"axes 2 and 1 in chunks of {chunk} planes through the "
                                   "Infinity Cache, then axis 0)" if chunk else
@@ -489,6 +519,7 @@ def main():
                    "stage_kernels": names,
                    "chunk_kernels": [describe(h) for h in cstages]},
         "hbm_tbs_algorithmic": round(tbs, 3),
+        "graph_replay_ms_per_step": round(graph_ms, 4) if graph_ms else None,
         "part_ms": ({"axes 2+1 (chunked)": round(part_ms[0], 4), "axis 0": round(part_ms[1], 4)}
                     if chunk else {f"axis {D - 1 - s}": round(t, 4) for s, t in enumerate(part_ms)}),
         "accuracy": acc,
