@@ -35,6 +35,22 @@ template <typename B> constexpr int nd_precision() {
   return std::is_same<typename B::FloatType, double>::value ? PFX_F64 : PFX_F32;
 }
 
+/// Measure / Patient planning of a one-rank N-D plan: the FFTW_MEASURE
+/// analogue for the fast path (kernels, and chunked or stage-by-stage
+/// schedule, timed on a temporary array-sized device buffer that is freed
+/// again; the user's arrays are never touched).
+inline void plan_nd(pfx_ndplan nd, FFTPlanFlag flag, std::size_t bytes, pfx_stream stream) {
+  if (flag == FFTPlanFlag::Estimate) return;
+  void *tmp = nullptr;
+  parafaft::check_pfx(pfx_malloc(&tmp, bytes), "pfx_malloc (planning buffer)");
+  const int rc = pfx_ndplan_tune(nd, flag == FFTPlanFlag::Measure ? PFX_PLAN_MEASURE
+                                                                 : PFX_PLAN_PATIENT,
+                                 tmp, stream);
+  pfx_stream_synchronize(stream);
+  pfx_free(tmp);
+  parafaft::check_pfx(rc, "pfx_ndplan_tune");
+}
+
 template <int D, typename Backend = parafaft::MI355Backend<double>>
 class ParaFaFT_C2C {
   static_assert(D >= 1, "D >= 1");
@@ -54,6 +70,7 @@ public:
       parafaft::check_pfx(pfx_ndplan_create(&nd_, nd_precision<Backend>(), PFX_C2C, D,
                                             global_shape_),
                           "pfx_ndplan_create");
+      plan_fast(flag, size_ * sizeof(Complex));
       return;
     }
     scratch_.resize(size_);
@@ -84,6 +101,7 @@ public:
   Backend &backend() { return backend_; }
 
 private:
+  void plan_fast(FFTPlanFlag flag, std::size_t bytes) { plan_nd(nd_, flag, bytes, backend_.stream()); }
   static void copy(int *dst, const int *src) {
     for (int i = 0; i < D; ++i) dst[i] = src[i];
   }
@@ -184,6 +202,7 @@ public:
     if (mode_ == Mode::Fast) {
       parafaft::check_pfx(pfx_ndplan_create(&nd_, nd_precision<Backend>(), PFX_R2C, D, real_),
                           "pfx_ndplan_create");
+      plan_fast(flag, get_required_output_size() * sizeof(FloatType));
       return;
     }
     scratch_.resize(csize_);
@@ -274,6 +293,7 @@ public:
   Backend &backend() { return backend_; }
 
 private:
+  void plan_fast(FFTPlanFlag flag, std::size_t bytes) { plan_nd(nd_, flag, bytes, backend_.stream()); }
   std::size_t cprod(int from, int to) const {
     std::size_t p = 1;
     for (int i = from; i < to; ++i) p *= static_cast<std::size_t>(cplx_[i]);

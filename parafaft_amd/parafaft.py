@@ -43,6 +43,19 @@ class _NdPlan:
         A.check(A.load().pfx_ndplan_execute(self.handle, direction, ptr(data), scale, stream),
                 "pfx_ndplan_execute")
 
+    def plan(self, flag: FFTPlanFlag, nbytes: int, stream):
+        """Measure / Patient planning (kernels and chunked or stage-by-stage
+        schedule) on a temporary device buffer of the array's size, freed
+        again: the user's arrays are never touched."""
+        if flag == FFTPlanFlag.Estimate:
+            return
+        tmp = DeviceBuffer(nbytes)
+        L = A.load()
+        rc = L.pfx_ndplan_tune(self.handle, flag.value, tmp.ptr, stream)
+        L.pfx_stream_synchronize(stream)
+        del tmp
+        A.check(rc, "pfx_ndplan_tune")
+
     def __del__(self):
         try:
             if self.handle:
@@ -69,6 +82,7 @@ class ParaFaFT_C2C:
         self.max_stage_size = _prod(self.global_shape)
         if fast:
             self._nd = _NdPlan(self.precision, A.PFX_C2C, self.global_shape)
+            self._nd.plan(plan_flag, self.max_stage_size * self.esz, self.backend.stream)
             self.scratch = None
         else:
             self.scratch = DeviceBuffer(self.max_stage_size * self.esz)
@@ -183,6 +197,8 @@ class ParaFaFT_R2C:
         self.max_complex_size = _prod(self.global_complex_shape)
         if fast:
             self._nd = _NdPlan(self.precision, A.PFX_R2C, self.global_real_shape)
+            self._nd.plan(plan_flag, self.get_required_output_size() * self.esz // 2,
+                          self.backend.stream)
         else:
             self.scratch = DeviceBuffer(self.max_complex_size * self.esz)
             self._create_backend_plans()

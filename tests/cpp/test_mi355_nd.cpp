@@ -9,6 +9,7 @@
 #include <cmath>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <vector>
 
 #include "orc.h"
@@ -31,7 +32,9 @@ static void sync_copy(void *dst, const void *src, size_t bytes) {
   parafaft::check_pfx(pfx_stream_synchronize(nullptr), "sync");
 }
 
-template <typename T, int D> static void c2c_case(int N, int shape_id, Mode mode) {
+template <typename T, int D>
+static void c2c_case(int N, int shape_id, Mode mode,
+                     parafaft::FFTPlanFlag flag = parafaft::FFTPlanFlag::Estimate) {
   int shape[D];
   size_t tot = 1;
   for (int i = 0; i < D; ++i) {
@@ -47,7 +50,7 @@ template <typename T, int D> static void c2c_case(int N, int shape_id, Mode mode
   for (auto &v : ref) peak = std::max(peak, std::abs(v));
 
   using B = parafaft::MI355Backend<T>;
-  parafaft_mi355::ParaFaFT_C2C<D, B> fft(shape, parafaft::FFTPlanFlag::Estimate, mode);
+  parafaft_mi355::ParaFaFT_C2C<D, B> fft(shape, flag, mode);
   typename B::ComplexBuffer d(fft.get_required_output_size());
   std::vector<std::complex<T>> h(tot);
   for (size_t i = 0; i < tot; ++i) h[i] = std::complex<T>(T(x[i]), 0);
@@ -65,7 +68,9 @@ template <typename T, int D> static void c2c_case(int N, int shape_id, Mode mode
   report(ok, "C2C", dbl ? "double" : "float", D, N, shape_id, mode, e1, e2);
 }
 
-template <typename T, int D> static void r2c_case(int N, int shape_id, Mode mode) {
+template <typename T, int D>
+static void r2c_case(int N, int shape_id, Mode mode,
+                     parafaft::FFTPlanFlag flag = parafaft::FFTPlanFlag::Estimate) {
   int shape[D];
   size_t rows = 1;
   for (int i = 0; i < D; ++i) shape[i] = N;
@@ -85,7 +90,7 @@ template <typename T, int D> static void r2c_case(int N, int shape_id, Mode mode
   for (double v : ref) peak = std::max(peak, std::abs(v));
 
   using B = parafaft::MI355Backend<T>;
-  parafaft_mi355::ParaFaFT_R2C<D, B> fft(shape, parafaft::FFTPlanFlag::Estimate, mode);
+  parafaft_mi355::ParaFaFT_R2C<D, B> fft(shape, flag, mode);
   typename B::Buffer d(fft.get_required_output_size());
   sync_copy(d.data(), pad.data(), pad.size() * sizeof(T));
   fft.forward_in_place(d.data());
@@ -103,7 +108,19 @@ template <typename T, int D> static void r2c_case(int N, int shape_id, Mode mode
   report(ok, "R2C", dbl ? "double" : "float", D, N, shape_id, mode, e1, e2);
 }
 
+// Fast mode with Measure planning (temporary planning buffer, kernels and
+// schedule timed); a 1 MB chunk budget makes 64^3 complex<double> and the
+// 4D case take the chunked schedule into the comparison.
+template <typename T> static void measured() {
+  setenv("PFX_ND_CHUNK_MB", "1", 1);
+  c2c_case<T, 3>(64, 1, Mode::Fast, parafaft::FFTPlanFlag::Measure);
+  c2c_case<T, 4>(16, 2, Mode::Fast, parafaft::FFTPlanFlag::Patient);
+  r2c_case<T, 3>(64, 0, Mode::Fast, parafaft::FFTPlanFlag::Measure);
+  unsetenv("PFX_ND_CHUNK_MB");
+}
+
 template <typename T> static void matrix() {
+  measured<T>();
   for (Mode m : {Mode::Faithful, Mode::Fast})
     for (int s = 0; s < 3; ++s) {
       c2c_case<T, 2>(32, s, m);

@@ -118,3 +118,33 @@ def test_nonpow2_c2c_and_r2c(shape):
         fft.forward_in_place(d)
         got = to_host(d).view(np.complex128)
         assert rel_l2(got, O.rfftn_padded(r)) <= 1e-13
+
+
+@pytest.mark.parametrize("kind", ["c2c", "r2c"])
+def test_fast_measure_planning(kind, monkeypatch):
+    """Fast mode with FFTPlanFlag.Measure: kernels and the chunked or
+    stage-by-stage schedule are timed on a temporary buffer at construction
+    (a 1 MB chunk budget brings the chunked schedule into the comparison);
+    the user's array is untouched until forward, and the result matches the
+    oracle."""
+    import parafaft_amd as P
+    from parafaft_amd.backend import FFTPlanFlag
+    monkeypatch.setenv("PFX_ND_CHUNK_MB", "1")
+    shape = [64, 48, 40]
+    if kind == "c2c":
+        x = O.uniform_complex(tuple(shape), seed=808)
+        d = to_dev(x)
+        fft = P.ParaFaFT_C2C(shape, precision="f64", plan_flag=FFTPlanFlag.Measure, fast=True)
+        assert np.array_equal(to_host(d), x)
+        fft.forward(d)
+        assert rel_l2(to_host(d), O.fftn(x)) <= 1e-13
+    else:
+        x = O.uniform(int(np.prod(shape)), seed=809).reshape(shape)
+        fft = P.ParaFaFT_R2C(shape, precision="f64", plan_flag=FFTPlanFlag.Measure, fast=True)
+        h = shape[-1] // 2 + 1
+        pad = np.zeros(fft.get_required_output_size())
+        pad.reshape(-1)[:np.prod(shape[:-1]) * 2 * h].reshape(shape[:-1] + [2 * h])[..., :shape[-1]] = x
+        d = to_dev(pad)
+        fft.forward_in_place(d)
+        got = to_host(d)[:np.prod(shape[:-1]) * 2 * h].view(np.complex128).reshape(shape[:-1] + [h])
+        assert rel_l2(got, O.rfftn_padded(x)) <= 1e-13
