@@ -55,19 +55,26 @@ struct SegMap {
 };
 
 // Address of element e of the transform at (lead, trail); a transform's
-// (lead, trail) is split from its index once, at set-up.  The peer is found
-// by a uniform scan with selects (no per-lane indexing into the map, which
-// would move the kernel arguments to scratch); the host guarantees
-// lead * n[p] + n[p] < 2^31 and trailing < 2^32.
-// Maps are read in place from the kernel-argument segment (constant address
-// space, scalar loads).
+// (lead, trail) is split from its index once, at set-up.  The lanes of a
+// wave that load or store together hold the consecutive elements e0 + j of
+// a block of `blk` elements (j < blk their position in the transform), with
+// e0 the same for the whole wave.  So the peer of e0 is found with scalar
+// work (loop and loads from the kernel-argument segment), and per lane only
+// the peer boundaries inside the block need a select; usually there are
+// none.  No per-lane indexing into the map (that would move the kernel
+// arguments to scratch).  The host guarantees lead * n[p] + n[p] < 2^31 and
+// trailing < 2^32.
 using KSegMap = const __attribute__((address_space(4))) SegMap;
 
 template <typename T>
-__device__ __forceinline__ V2<T>* seg_addr(KSegMap* m, uint32_t lead, int64_t trail, int e) {
-  V2<T>* base = static_cast<V2<T>*>(m->seg[0]);
-  int nn = m->n[0], ss = 0;
-  for (int q = 1; q < m->nparts; ++q) {
+__device__ __forceinline__ V2<T>* seg_addr(KSegMap* m, uint32_t lead, int64_t trail, int e,
+                                           int e0, int blk) {
+  int p0 = 0;
+  for (int q = 1; q < m->nparts; ++q)
+    if (e0 >= m->s[q]) p0 = q;
+  V2<T>* base = static_cast<V2<T>*>(m->seg[p0]);
+  int nn = m->n[p0], ss = m->s[p0];
+  for (int q = p0 + 1; q < m->nparts && m->s[q] < e0 + blk; ++q) {
     const bool here = e >= m->s[q];
     base = here ? static_cast<V2<T>*>(m->seg[q]) : base;
     nn = here ? m->n[q] : nn;
@@ -203,12 +210,18 @@ template <typename T> struct GlobalPtr<T, true> {
   KSegMap* out_map;          // non-null: stores go to a segment layout
   uint32_t in_lead, out_lead;  // transform's place in each map
   int64_t in_trail, out_trail;
+  int lane_j;                // this lane's position among its transform's threads
+  int blk;                   // threads per transform
+  // first element of the wave's block (e - lane_j is the same on every lane)
+  __device__ __forceinline__ int block_start(int e) const {
+    return __builtin_amdgcn_readfirstlane(e - lane_j);
+  }
   __device__ __forceinline__ V2<T> load(int e) const {
-    if (in_map) return *seg_addr<T>(in_map, in_lead, in_trail, e);
+    if (in_map) return *seg_addr<T>(in_map, in_lead, in_trail, e, block_start(e), blk);
     return pin[(int64_t)e * stride];
   }
   __device__ __forceinline__ void store(int e, V2<T> v) const {
-    if (out_map) *seg_addr<T>(out_map, out_lead, out_trail, e) = v;
+    if (out_map) *seg_addr<T>(out_map, out_lead, out_trail, e, block_start(e), blk) = v;
     else p[(int64_t)e * stride] = v;
   }
 };
@@ -367,6 +380,10 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
   const int64_t b = tile * C + c;
   GlobalPtr<T, SEG> g;
   init_global(g, a, b);
+  if constexpr (SEG) {
+    g.lane_j = j;
+    g.blk = TPT;
+  }
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
@@ -399,6 +416,10 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   const int64_t b = tile * C + c;
   GlobalPtr<T, SEG> g;
   init_global(g, a, b);
+  if constexpr (SEG) {
+    g.lane_j = j;
+    g.blk = TPT;
+  }
   g.stride = 1;
   g.swap_io = false;
   g.scale = T(a.scale);
@@ -479,6 +500,10 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
   const int64_t b = tile * C + c;
   GlobalPtr<T, SEG> g;
   init_global(g, a, b);
+  if constexpr (SEG) {
+    g.lane_j = j;
+    g.blk = TPT;
+  }
   g.stride = 1;
   g.swap_io = true;  // inverse FFT via re/im swap; pre-twiddled data is stored swapped
   g.scale = T(a.scale);
@@ -644,6 +669,10 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
   const int64_t b = tile * C + c;
   GlobalPtr<T, SEG> g;
   init_global(g, a, b);
+  if constexpr (SEG) {
+    g.lane_j = j;
+    g.blk = TPT;
+  }
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
