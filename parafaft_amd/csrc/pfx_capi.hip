@@ -317,6 +317,8 @@ static std::vector<const KernelEntry *> candidates(const pfx_plan_t *p) {
   for (const KernelEntry *e : registry().entries) {
     if (e->kind != k0->kind || e->n != k0->n || e->col != k0->col) continue;
     if (e->prec != k0->prec) continue;  // same arithmetic (plain or the same split form)
+    // one launch must fit the grid limit with this tile
+    if ((p->inner * p->outer + e->per_block - 1) / e->per_block > 0x7fffffff) continue;
     out.push_back(e);
   }
   return out;
