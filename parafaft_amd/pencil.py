@@ -79,20 +79,23 @@ class PencilShareR2C:
             axis = D - 1 - stage
             length = self.gc[axis]
             shp = ins[stage]
+            # (strides only matter when there are transforms; an empty share,
+            # where decompose gives this rank no elements, has none)
             if axis == 0:
                 batch = math.prod(shp[1:])
-                self.plans.append(Plan.c2c(self.precision, length, batch, batch, 1))
+                self.plans.append(Plan.c2c(self.precision, length, batch, max(1, batch), 1))
                 self.leading.append(1)
                 self.slice.append(0)
             else:
                 trailing = math.prod(shp[axis + 1:])
                 leading = math.prod(shp[:axis])
                 if fast:
-                    self.plans.append(Plan.c2c(self.precision, length, trailing, trailing, 1,
-                                               leading, length * trailing))
+                    self.plans.append(Plan.c2c(self.precision, length, trailing, max(1, trailing),
+                                               1, leading, length * trailing))
                     self.leading.append(1)
                 else:
-                    self.plans.append(Plan.c2c(self.precision, length, trailing, trailing, 1))
+                    self.plans.append(Plan.c2c(self.precision, length, trailing, max(1, trailing),
+                                               1))
                     self.leading.append(leading)
                 self.slice.append(length * trailing * self.esz)
         # the fused form needs one launch per stage (fast plans), tiled

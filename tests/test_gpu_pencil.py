@@ -226,6 +226,10 @@ FUSED_CASES = [
     ([96, 80, 64], [2, 3], [1, 2]),
     ([64, 256], [4], [1]),
     ([32, 16, 32, 64], [2, 2, 2], [1, 0, 1]),
+    # complex last axis 3 over 4 ranks: blocks 1, 1, 1, 0 (an empty peer block)
+    ([8, 8, 4], [4, 4], [1, 2]),
+    # ... and this rank's own complex share is empty
+    ([8, 8, 4], [4, 4], [1, 3]),
 ]
 
 

@@ -239,3 +239,23 @@ def test_wide_4096_column_kernel(monkeypatch):
         d = to_dev(x)
         plan.execute(direction, d)
         assert rel_l2(to_host(d), _ref_c2c(x, n, batch, batch, 1, sign)) <= tol("f32", n)
+
+
+@pytest.mark.parametrize("kind", ["c2c", "r2c", "c2r"])
+def test_empty_batch(kind):
+    """A plan over zero transforms executes as a no-op (nothing launched,
+    nothing touched)."""
+    import parafaft_amd as P
+    from parafaft_amd import _capi as A
+    x = np.arange(64, dtype=np.float32)
+    d = to_dev(x)
+    if kind == "c2c":
+        plan = P.Plan.c2c("f32", 16, 0, 1, 16)
+        plan.execute(A.PFX_FORWARD, d)
+    elif kind == "r2c":
+        plan = P.Plan.r2c("f32", 16, 0, 1, 18)
+        plan.execute(A.PFX_FORWARD, d)
+    else:
+        plan = P.Plan.c2r("f32", 16, 0, 1, 18)
+        plan.execute(A.PFX_BACKWARD, d)
+    assert np.array_equal(to_host(d), x)
