@@ -6,9 +6,10 @@
 // v_mfma_f32_16x16x16 (FP32 accumulation).  FP32 accuracy is won back by
 // splitting both operands:
 //   FP16: x = hi + lo (hi = fp16(x), lo = fp16(x - hi)), keeping the three
-//         products hi*hi + hi*lo + lo*hi; before the split every group of 16
-//         butterflies is scaled by a power of two so its largest magnitude
-//         sits just under 2^15 (exact, undone on the FP32 accumulators).
+//         products hi*hi + hi*lo + lo*hi; before the split the 16 inputs
+//         of every butterfly are scaled by a power of two so their largest
+//         magnitude sits just under 2^15 (exact, undone on the FP32
+//         accumulators).
 //   BF16: x = hi + mid + lo (8-bit mantissas each), keeping the six products
 //         of total order <= 2; BF16 has FP32's range, so no scaling.
 // Twiddles between passes and passes of other radices stay in FP32.
@@ -42,18 +43,28 @@ __device__ __forceinline__ float bf16_to_f(short h) {
   return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
 }
 
-__device__ __forceinline__ float wave_max(float v) {
-#pragma unroll
-  for (int o = 32; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-  return v;
+// Largest value over the 16 inputs of the butterfly in lane column
+// u = lane % 16 (lanes u, u + 16, u + 32, u + 48 hold them).
+__device__ __forceinline__ float butterfly_max(float v) {
+  v = fmaxf(v, __shfl_xor(v, 16, 64));
+  return fmaxf(v, __shfl_xor(v, 32, 64));
 }
 
-// cos / sin of 2 pi t / 16, t = 0..15
+// cos(2 pi t / 16), t = 0..15, by selects (no indexed table, which would
+// live in scratch memory)
+__device__ __forceinline__ float cos16(int t) {
+  int a = t & 15;
+  if (a > 8) a = 16 - a;  // cos(2 pi (16 - a) / 16) = cos(2 pi a / 16)
+  const bool neg = a > 4;
+  if (neg) a = 8 - a;     // cos(pi - x) = -cos x
+  const float c = a == 0 ? 1.f : a == 1 ? float(k::COS_PI8) : a == 2 ? float(k::SQRT1_2)
+                : a == 3 ? float(k::SIN_PI8) : 0.f;
+  return neg ? -c : c;
+}
+// cos / sin of 2 pi t / 16
 __device__ __forceinline__ void root16(int t, float& c, float& s) {
-  const float a = float(k::COS_PI8), b = float(k::SIN_PI8), r = float(k::SQRT1_2);
-  const float cs[16] = {1.f, a, r, b, 0.f, -b, -r, -a, -1.f, -a, -r, -b, 0.f, b, r, a};
-  c = cs[t & 15];
-  s = cs[(t + 12) & 15];  // sin x = cos(x - pi/2)
+  c = cos16(t);
+  s = cos16(t + 12);  // sin x = cos(x - pi/2)
 }
 
 template <int MODE> struct MfmaDft16;
@@ -78,16 +89,21 @@ template <> struct MfmaDft16<SPLIT_F16> {
     }
   }
   // x: this lane's 4 inputs (m = 4 (lane/16) + r); y: its 4 outputs.
-  // Every lane of the wave must call this together.
-  __device__ void apply(const float2* x, float2* y) const {
+  // Every lane of the wave must call this together.  Each butterfly gets its
+  // own power-of-two scale, so precision is relative to the butterfly's
+  // largest input.
+  __device__ __forceinline__ void apply(const float2* x, float2* y) const {
     float amax = 0.f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) amax = fmaxf(amax, fmaxf(fabsf(x[r].x), fabsf(x[r].y)));
-    amax = wave_max(amax);
-    int e = 0;
-    if (amax > 0.f) frexpf(amax, &e);  // amax < 2^e
-    const int sh = e - 15;             // scaled values below 2^15
-    const float sin_ = ldexpf(1.f, -sh), sout = ldexpf(1.f, sh);
+    amax = butterfly_max(amax);
+    // amax < 2^ex, from the exponent field (ex = -126 for zero and
+    // subnormals); the shift keeps the scaled maximum below 2^15 and both
+    // factors normal
+    const int ex = (int)(__float_as_uint(amax) >> 23) - 126;
+    const int sh = ex - 15 < -100 ? -100 : ex - 15;
+    const float sin_ = __uint_as_float((uint32_t)(127 - sh) << 23);
+    const float sout = __uint_as_float((uint32_t)(127 + sh) << 23);
     half4_t xrh, xrl, xih, xil;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -132,6 +148,7 @@ template <> struct MfmaDft16<SPLIT_BF16> {
       float c, s;
       root16(kout * (m0 + r), c, s);
       const float vals[3] = {c, -s, s};
+#pragma unroll
       for (int q = 0; q < 3; ++q) {
         short h, m, l;
         split3(vals[q], h, m, l);
@@ -152,7 +169,7 @@ template <> struct MfmaDft16<SPLIT_BF16> {
     z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b[0], z, 0, 0, 0);
     return z;
   }
-  __device__ void apply(const float2* x, float2* y) const {
+  __device__ __forceinline__ void apply(const float2* x, float2* y) const {
     bf16x4_t xr[3], xi[3];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -286,6 +303,230 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
     if (a.use_scale) x = scl<float>(x, sc);
     if (sw) x = swp<float>(x);
     if (valid) p[(int64_t)e * a.stride] = x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Register-resident split kernels for N = 256 R3 (passes 16, 16, R3 with
+// R3 = 1, 2, 4 or 8).  Both radix-16 passes run on the matrix cores with the
+// butterfly's 16 inputs in the B operand (lane l holds inputs 4 (l/16) + r,
+// r < 4, of the butterfly in lane column u = l % 16, and receives the same
+// outputs):
+//  - pass 1 loads straight from HBM into that layout;
+//  - one exchange through LDS regroups its outputs for pass 2;
+//  - a lane holds the pass-2 butterflies b + 16 g, g < R3, whose outputs m
+//    are exactly the R3 inputs of pass-3 butterfly b + 16 m, so pass 3 runs
+//    in registers and stores straight to HBM.
+// Pass 2 (span 16) reads twiddle W_256^{b m}, pass 3 (span 256) W_N^{jj g},
+// both from the per-pass table in global memory (entry m Ns + kk - 1).
+// ---------------------------------------------------------------------------
+template <int R3> constexpr int log2_of() { return R3 == 1 ? 0 : R3 == 2 ? 1 : R3 == 4 ? 2 : 3; }
+
+// Pass 3 (radix R3, span 256) on values a lane holds: v[g] is input g of
+// butterfly jj; twiddle W_N^{jj g}, then the butterfly; output g lands at
+// jj + 256 g.
+template <int R3>
+__device__ __forceinline__ void split_last_pass(V2<float>* v, const V2<float>* tw, int jj) {
+  if constexpr (R3 > 1) {
+#pragma unroll
+    for (int g = 1; g < R3; ++g) v[g] = cmul<float>(v[g], tw[g * 256 + jj - 1]);
+    bfly<float, R3>(v);
+  }
+}
+
+// Row layout: one wave per transform, lane column u indexes 16 adjacent
+// butterflies, and the exchange stays inside the wave (no workgroup barrier).
+// LDS slot of element p: p with bits 0-1 and 4 XOR-ed by bits 5-6 and L of
+// p, so both the pass-1 stores (p = 256 g + 16 u + 4 q + r) and the pass-2
+// loads (p = 16 g + u + (N/16)(4 q + r)) spread a wave's 64 lanes over all
+// 32 complex slots of a bank row, two lanes each.
+template <int R3> __device__ __forceinline__ int split_row_slot(int p) {
+  constexpr int L = 6 + log2_of<R3>();
+  return p ^ ((p >> 5) & 3) ^ (((p >> L) & 1) << 4);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int MODE, int R3, int WPB>
+__global__ void __launch_bounds__(64 * WPB) c2c_split_row_kernel(StockhamArgs a) {
+  constexpr int N = 256 * R3, NB = N / 16;
+  __shared__ __align__(16) V2<float> lds_all[WPB * N];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u = lane & 15, q = lane >> 4;
+  V2<float>* lds = lds_all + wave * N;
+  GlobalPtr<float> g;
+  init_global(g, a, (int64_t)blockIdx.x * WPB + wave);
+  g.stride = a.stride;
+  const bool sw = a.backward != 0;
+  const V2<float>* tw = static_cast<const V2<float>*>(a.ptw);
+  const MfmaDft16<MODE> dft;
+  V2<float> v[R3][4];
+  // pass 1: butterfly 16 i + u, input m = 4 q + r at element 16 i + u + NB m
+#pragma unroll
+  for (int i = 0; i < R3; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      V2<float> x = mk<float>(0.f, 0.f);
+      if (g.valid) x = g.load(16 * i + u + NB * (4 * q + r));
+      v[i][r] = sw ? swp<float>(x) : x;
+    }
+#pragma unroll
+  for (int i = 0; i < R3; ++i) {
+    V2<float> y[4];
+    dft.apply(v[i], y);
+    // output m of butterfly jj lands at 16 jj + m
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lds[split_row_slot<R3>(256 * i + 16 * u + 4 * q + r)] = y[r];
+  }
+  wave_lds_sync();
+  // pass 2: butterfly 16 i + u (span 16), input m = 4 q + r at 16 i + u + NB m
+  V2<float> t2[4];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int m = 4 * q + r;
+    t2[r] = m > 0 ? tw[m * 16 + u - 1] : mk<float>(1.f, 0.f);
+  }
+#pragma unroll
+  for (int i = 0; i < R3; ++i) {
+    V2<float> x[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+      x[r] = cmul<float>(lds[split_row_slot<R3>(16 * i + u + NB * (4 * q + r))], t2[r]);
+    dft.apply(x, v[i]);
+  }
+  // pass 3: butterfly jj = u + 16 m holds output m of pass-2 butterflies
+  // 16 i + u, i < R3, which land at 256 i + u + 16 m
+  const float sc = (float)a.scale;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const int jj = u + 16 * (4 * q + r);
+    V2<float> t[R3];
+#pragma unroll
+    for (int i = 0; i < R3; ++i) t[i] = v[i][r];
+    split_last_pass<R3>(t, tw, jj);
+#pragma unroll
+    for (int i = 0; i < R3; ++i) {
+      V2<float> x = t[i];
+      if (a.use_scale) x = scl<float>(x, sc);
+      if (sw) x = swp<float>(x);
+      if (g.valid) g.store(jj + 256 * i, x);
+    }
+  }
+}
+
+// Batch-contiguous layout: lane column u indexes 16 adjacent transforms (a
+// 128-byte run per element), W waves share a group of 16 transforms and CG
+// groups make a workgroup.  The exchange goes through an LDS tile of
+// N x 16 CG floats, one component at a time; element e of transform c sits
+// at e * C + c with the four 16-float quarters of every 64-float bank row
+// permuted by bits 2-3 and L-(L+1) of e, which keeps the pass-1 stores
+// (e = 16 jj + 4 q + r) and the pass-2 loads (e = jj + (N/16)(4 q + r)) free
+// of bank conflicts.
+template <int N, int C> struct SplitColLds {
+  static constexpr int L = (N == 256 ? 6 : N == 512 ? 7 : N == 1024 ? 8 : 9);  // log2(N / 4)
+  static constexpr int size = N * C;
+  __device__ static __forceinline__ int at(int e, int c) {
+    const int key = ((e >> 2) ^ (e >> L)) & 3;
+    return (e * C + c) ^ (key << 4);
+  }
+};
+
+template <int MODE, int R3, int W, int CG>
+__global__ void __launch_bounds__(64 * W * CG) c2c_split_col_kernel(StockhamArgs a) {
+  constexpr int N = 256 * R3, NB = N / 16, C = 16 * CG;
+  constexpr int P1 = NB / W;  // pass-1 butterflies per lane
+  constexpr int P2 = 16 / W;  // pass-2 butterfly columns per lane
+  static_assert(16 % W == 0 && P1 == P2 * R3, "waves must split the butterflies evenly");
+  using L = SplitColLds<N, C>;
+  __shared__ __align__(16) float lds[L::size];
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u = lane & 15, q = lane >> 4;
+  const int cg = wave % CG, w = wave / CG;
+  const int c = 16 * cg + u;
+  const int64_t tile = tile_order(blockIdx.x, gridDim.x, a.xcd_remap);
+  GlobalPtr<float> g;
+  init_global(g, a, tile * C + c);
+  g.stride = a.stride;
+  const bool sw = a.backward != 0;
+  const V2<float>* tw = static_cast<const V2<float>*>(a.ptw);
+  const MfmaDft16<MODE> dft;
+  V2<float> v[P1][4];
+  // pass 1: butterfly w + W i, input m = 4 q + r at element jj + NB m
+#pragma unroll
+  for (int i = 0; i < P1; ++i)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      V2<float> x = mk<float>(0.f, 0.f);
+      if (g.valid) x = g.load(w + W * i + NB * (4 * q + r));
+      v[i][r] = sw ? swp<float>(x) : x;
+    }
+#pragma unroll
+  for (int i = 0; i < P1; ++i) {
+    V2<float> y[4];
+    dft.apply(v[i], y);
+#pragma unroll
+    for (int r = 0; r < 4; ++r) v[i][r] = y[r];
+  }
+  // exchange: pass-1 output m of butterfly jj sits at 16 jj + m; pass 2
+  // takes butterflies b + 16 k2, b = w + W k, k2 < R3, at slot k R3 + k2
+#pragma unroll
+  for (int comp = 0; comp < 2; ++comp) {
+#pragma unroll
+    for (int i = 0; i < P1; ++i)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        lds[L::at(16 * (w + W * i) + 4 * q + r, c)] = comp ? v[i][r].y : v[i][r].x;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < P2; ++k)
+#pragma unroll
+      for (int k2 = 0; k2 < R3; ++k2)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const float s = lds[L::at(w + W * k + 16 * k2 + NB * (4 * q + r), c)];
+          if (comp) v[k * R3 + k2][r].y = s;
+          else v[k * R3 + k2][r].x = s;
+        }
+    __syncthreads();
+  }
+  const float sc = (float)a.scale;
+#pragma unroll
+  for (int k = 0; k < P2; ++k) {
+    const int b = w + W * k;
+    // pass 2 (span 16): twiddle W_256^{b m}
+    V2<float> t2[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = 4 * q + r;
+      t2[r] = m > 0 ? tw[m * 16 + b - 1] : mk<float>(1.f, 0.f);
+    }
+#pragma unroll
+    for (int k2 = 0; k2 < R3; ++k2) {
+      V2<float> x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r) x[r] = cmul<float>(v[k * R3 + k2][r], t2[r]);
+      dft.apply(x, v[k * R3 + k2]);
+    }
+    // pass 3 in registers, then HBM
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int jj = b + 16 * (4 * q + r);
+      V2<float> t[R3];
+#pragma unroll
+      for (int k2 = 0; k2 < R3; ++k2) t[k2] = v[k * R3 + k2][r];
+      split_last_pass<R3>(t, tw, jj);
+#pragma unroll
+      for (int k2 = 0; k2 < R3; ++k2) {
+        V2<float> x = t[k2];
+        if (a.use_scale) x = scl<float>(x, sc);
+        if (sw) x = swp<float>(x);
+        if (g.valid) g.store(jj + 256 * k2, x);
+      }
+    }
   }
 }
 

@@ -1,5 +1,5 @@
 """Split-precision MFMA form: radix-16 sub-DFTs on v_mfma_f32_16x16x16 with
-FP16 hi/lo (per-group power-of-two scaling) or BF16 hi/mid/lo operand splits,
+FP16 hi/lo (per-butterfly power-of-two scaling) or BF16 hi/mid/lo operand splits,
 FP32 accumulation; every result against the FP64 oracle, at the FP32 bound
 rel-L2 <= 1e-6 * log2(N)."""
 import numpy as np
@@ -39,7 +39,7 @@ def test_split_1d(mode, n, layout):
 @pytest.mark.parametrize("mode", MODES)
 def test_split_wide_dynamic_range(mode):
     """Inputs spanning 2^-20 .. 2^20 within one transform (the FP16 form's
-    per-group scaling keeps the large values finite)."""
+    per-butterfly scaling keeps the large values finite)."""
     import parafaft_amd as P
     n, batch = 1024, 8
     rng = np.random.default_rng(5)
@@ -67,3 +67,64 @@ def test_split_2d_through_fast_path(mode):
     assert rel_l2(to_host(d), ref) <= tol("f32", 256 * n)
     fft.backward(d, scale=1.0 / (256 * n))
     assert rel_l2(to_host(d), x) <= tol("f32", 256 * n)
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("n", [256, 512, 1024, 2048])
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_split_every_candidate(mode, n, layout):
+    """Every split kernel the planner may pick (LDS-staged and register-
+    resident variants), both directions, with a fused scale, on a batch that
+    leaves partial workgroups."""
+    import parafaft_amd as P
+    batch = 37
+    x = O.uniform_complex((n * batch,), seed=n + 7).astype(np.complex64)
+    stride, dist = (1, n) if layout == "row" else (batch, 1)
+    plan = P.Plan.c2c(mode, n, batch, stride, dist)
+    names = []
+    for i in range(plan.num_candidates()):
+        plan.use_candidate(i)
+        name = plan.describe()[0]
+        names.append(name)
+        for direction, sign, scale in ((0, -1, 1.0), (1, +1, 1.0 / n)):
+            d = to_dev(x)
+            plan.execute(direction, d, scale=scale)
+            ref = x.astype(np.complex128).copy()
+            O.c2c_batched(ref, n, batch, stride, dist, sign)
+            err = rel_l2(to_host(d), ref * scale)
+            assert err <= tol("f32", n), (name, direction, err)
+    # register-resident kernels exist for every case but 2048 batch-contiguous
+    # (32 values per lane at 1024 threads would spill)
+    assert any("split-reg" in nm for nm in names) == (layout == "row" or n < 2048), names
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("layout", ["row", "col"])
+def test_split_zero_and_tiny_butterflies(mode, layout):
+    """Butterflies whose 16 inputs are all zero (the scale's zero case) or
+    far below FP16's normal range sit next to unit-size ones."""
+    import parafaft_amd as P
+    n, batch = 1024, 32
+    rng = np.random.default_rng(11)
+    x = (rng.standard_normal((batch, n)) + 1j * rng.standard_normal((batch, n)))
+    x[0] = 0.0
+    x[1] *= 1e-30
+    x[2, ::2] = 0.0
+    x = x.astype(np.complex64)
+    if layout == "row":
+        flat, stride, dist = x.reshape(-1), 1, n
+    else:
+        flat, stride, dist = np.ascontiguousarray(x.T).reshape(-1), batch, 1
+    plan = P.Plan.c2c(mode, n, batch, stride, dist)
+    d = to_dev(flat)
+    plan.execute(0, d)
+    got = to_host(d)
+    ref = flat.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, stride, dist, -1)
+    assert np.all(np.isfinite(got))
+    assert rel_l2(got, ref) <= tol("f32", n)
+    g2 = got.reshape(batch, n) if layout == "row" else got.reshape(n, batch).T
+    r2 = ref.reshape(batch, n) if layout == "row" else ref.reshape(n, batch).T
+    assert np.all(g2[0] == 0)
+    # the tiny transform keeps FP32-level accuracy relative to itself
+    assert rel_l2(g2[1], r2[1]) <= tol("f32", n)
