@@ -14,6 +14,17 @@
 //         of total order <= 2; BF16 has FP32's range, so no scaling.
 // Twiddles between passes and passes of other radices stay in FP32.
 //
+// The matrix cores' FP32 accumulation rounds with a bias toward -infinity
+// (measured: the mean error of every output over a large batch is negative,
+// at tens of standard errors, where plain FP32 arithmetic shows none).  A
+// bias that is the same for every transform adds up coherently in a later
+// axis' sums: in a 1024^3 transform the low-frequency bins of the axis-0/1
+// DC line came out ~1e-4 off.  Each butterfly therefore runs on negated
+// inputs and negates its outputs when the parity of the set bits of its
+// transform and butterfly indices is odd (a Thue-Morse sign): the bias flips
+// sign with it, and cancels over any aligned power-of-two range of either
+// index.  For FP16 the sign rides on the power-of-two scale factors.
+//
 // Operand layout of v_mfma_f32_16x16x16 (A: M x K, B: K x N, D: M x N, lane l):
 //   A[i][k]: i = l % 16, k = 4 (l / 16) + r   (r = 0..3, one 4-vector per lane)
 //   B[k][n]: n = l % 16, k = 4 (l / 16) + r
@@ -67,6 +78,11 @@ __device__ __forceinline__ void root16(int t, float& c, float& s) {
   s = cos16(t + 12);  // sin x = cos(x - pi/2)
 }
 
+// Thue-Morse sign of a butterfly's pass through the matrix cores.
+__device__ __forceinline__ bool split_sign(int64_t b, int jj) {
+  return ((__popcll((unsigned long long)b) + __popc((unsigned)jj)) & 1) != 0;
+}
+
 template <int MODE> struct MfmaDft16;
 
 template <> struct MfmaDft16<SPLIT_F16> {
@@ -92,7 +108,7 @@ template <> struct MfmaDft16<SPLIT_F16> {
   // Every lane of the wave must call this together.  Each butterfly gets its
   // own power-of-two scale, so precision is relative to the butterfly's
   // largest input.
-  __device__ __forceinline__ void apply(const float2* x, float2* y) const {
+  __device__ __forceinline__ void apply(const float2* x, float2* y, bool neg) const {
     float amax = 0.f;
 #pragma unroll
     for (int r = 0; r < 4; ++r) amax = fmaxf(amax, fmaxf(fabsf(x[r].x), fabsf(x[r].y)));
@@ -102,8 +118,9 @@ template <> struct MfmaDft16<SPLIT_F16> {
     // factors normal
     const int ex = (int)(__float_as_uint(amax) >> 23) - 126;
     const int sh = ex - 15 < -100 ? -100 : ex - 15;
-    const float sin_ = __uint_as_float((uint32_t)(127 - sh) << 23);
-    const float sout = __uint_as_float((uint32_t)(127 + sh) << 23);
+    const uint32_t sbit = neg ? 0x80000000u : 0u;
+    const float sin_ = __uint_as_float(((uint32_t)(127 - sh) << 23) | sbit);
+    const float sout = __uint_as_float(((uint32_t)(127 + sh) << 23) | sbit);
     half4_t xrh, xrl, xih, xil;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -169,16 +186,17 @@ template <> struct MfmaDft16<SPLIT_BF16> {
     z = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[0], b[0], z, 0, 0, 0);
     return z;
   }
-  __device__ __forceinline__ void apply(const float2* x, float2* y) const {
+  __device__ __forceinline__ void apply(const float2* x, float2* y, bool neg) const {
+    const uint32_t sbit = neg ? 0x80000000u : 0u;
     bf16x4_t xr[3], xi[3];
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       short h, m, l;
-      split3(x[r].x, h, m, l);
+      split3(__uint_as_float(__float_as_uint(x[r].x) ^ sbit), h, m, l);
       xr[0][r] = h;
       xr[1][r] = m;
       xr[2][r] = l;
-      split3(x[r].y, h, m, l);
+      split3(__uint_as_float(__float_as_uint(x[r].y) ^ sbit), h, m, l);
       xi[0][r] = h;
       xi[1][r] = m;
       xi[2][r] = l;
@@ -189,14 +207,17 @@ template <> struct MfmaDft16<SPLIT_BF16> {
     zi = prod(f[0], xi, zi);  // Fr Xi
     zi = prod(f[1], xr, zi);  // Fi Xr
 #pragma unroll
-    for (int r = 0; r < 4; ++r) y[r] = mk<float>(zr[r], zi[r]);
+    for (int r = 0; r < 4; ++r)
+      y[r] = mk<float>(__uint_as_float(__float_as_uint(zr[r]) ^ sbit),
+                       __uint_as_float(__float_as_uint(zi[r]) ^ sbit));
   }
 };
 
 // One radix-16 Stockham pass over the workgroup's LDS tile on the matrix
 // cores: groups of 16 butterflies of one transform per MFMA product.
 template <int MODE, int N, int TPT, int C, bool COL, int Ns>
-__device__ __forceinline__ void mfma_pass16(V2<float>* lds, const V2<float>* stw, const MfmaDft16<MODE>& dft) {
+__device__ __forceinline__ void mfma_pass16(V2<float>* lds, const V2<float>* stw, const MfmaDft16<MODE>& dft,
+                                            int64_t b0) {
   using L = LdsLayout<float, N, C, COL>;
   constexpr int NB = N / 16;     // butterflies per transform
   constexpr int GPT = NB / 16;   // groups per transform
@@ -227,7 +248,7 @@ __device__ __forceinline__ void mfma_pass16(V2<float>* lds, const V2<float>* stw
     const int g = wave + gi * NW;
     const int c = g / GPT, jj = (g % GPT) * 16 + (lane & 15);
     float2 y[4];
-    dft.apply(xin[gi], y);
+    dft.apply(xin[gi], y, split_sign(b0 + c, jj));
     const int ob = (jj / Ns) * Ns * 16 + jj % Ns;
 #pragma unroll
     for (int r = 0; r < 4; ++r) lds[L::at(ob + (m0 + r) * Ns, c)] = y[r];
@@ -240,16 +261,18 @@ struct SplitPasses;
 
 template <int MODE, int N, int TPT, int C, bool COL, int Ns>
 struct SplitPasses<MODE, N, TPT, C, COL, Ns> {
-  __device__ static __forceinline__ void run(int, int, V2<float>*, const V2<float>*, const MfmaDft16<MODE>&) {}
+  __device__ static __forceinline__ void run(int, int, V2<float>*, const V2<float>*,
+                                             const MfmaDft16<MODE>&, int64_t) {}
 };
 
 template <int MODE, int N, int TPT, int C, bool COL, int Ns, int R, int... Rest>
 struct SplitPasses<MODE, N, TPT, C, COL, Ns, R, Rest...> {
   using L = LdsLayout<float, N, C, COL>;
+  // b0: index of the workgroup's first transform
   __device__ static __forceinline__ void run(int j, int c, V2<float>* lds, const V2<float>* stw,
-                                    const MfmaDft16<MODE>& dft) {
+                                    const MfmaDft16<MODE>& dft, int64_t b0) {
     if constexpr (R == 16) {
-      mfma_pass16<MODE, N, TPT, C, COL, Ns>(lds, stw, dft);
+      mfma_pass16<MODE, N, TPT, C, COL, Ns>(lds, stw, dft, b0);
     } else {
       auto load = [&](int e) -> V2<float> { return lds[L::at(e, c)]; };
       struct StoreLds {
@@ -261,7 +284,7 @@ struct SplitPasses<MODE, N, TPT, C, COL, Ns, R, Rest...> {
       };
       stockham_pass<float, N, R, Ns, TPT>(j, stw, load, StoreLds{lds, c});
     }
-    SplitPasses<MODE, N, TPT, C, COL, Ns * R, Rest...>::run(j, c, lds, stw, dft);
+    SplitPasses<MODE, N, TPT, C, COL, Ns * R, Rest...>::run(j, c, lds, stw, dft, b0);
   }
 };
 
@@ -293,7 +316,7 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
     lds[L::at(e, c)] = sw ? swp<float>(x) : x;
   }
   __syncthreads();
-  SplitPasses<MODE, N, TPT, C, COL, 1, Rs...>::run(j, c, lds, tw, dft);
+  SplitPasses<MODE, N, TPT, C, COL, 1, Rs...>::run(j, c, lds, tw, dft, tile * C);
   // LDS -> HBM
   const float sc = (float)a.scale;
 #pragma unroll
@@ -319,6 +342,9 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
 //    in registers and stores straight to HBM.
 // Pass 2 (span 16) reads twiddle W_256^{b m}, pass 3 (span 256) W_N^{jj g},
 // both from the per-pass table in global memory (entry m Ns + kk - 1).
+// The direction is a template parameter of the body (one uniform branch per
+// kernel instead of a select per element), and every unrolled load and store
+// offset is a fixed combination of a few precomputed multiples of the stride.
 // ---------------------------------------------------------------------------
 template <int R3> constexpr int log2_of() { return R3 == 1 ? 0 : R3 == 2 ? 1 : R3 == 4 ? 2 : 3; }
 
@@ -334,6 +360,19 @@ __device__ __forceinline__ void split_last_pass(V2<float>* v, const V2<float>* t
   }
 }
 
+// Pass-2 twiddle W_256^{b m} for input m (1 for m = 0; the table index is
+// kept in range for every lane).
+__device__ __forceinline__ V2<float> split_pass2_twiddle(const V2<float>* tw, int m, int b) {
+  const V2<float> t = tw[m > 0 ? m * 16 + b - 1 : 0];
+  return m > 0 ? t : mk<float>(1.f, 0.f);
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Row layout: one wave per transform, lane column u indexes 16 adjacent
 // butterflies, and the exchange stays inside the wave (no workgroup barrier).
 // LDS slot of element p: p with bits 0-1 and 4 XOR-ed by bits 5-6 and L of
@@ -345,39 +384,32 @@ template <int R3> __device__ __forceinline__ int split_row_slot(int p) {
   return p ^ ((p >> 5) & 3) ^ (((p >> L) & 1) << 4);
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-template <int MODE, int R3, int WPB>
-__global__ void __launch_bounds__(64 * WPB) c2c_split_row_kernel(StockhamArgs a) {
+template <int MODE, int R3, int WPB, bool SW>
+__device__ __forceinline__ void split_row_body(const StockhamArgs& a, V2<float>* lds, int64_t b) {
   constexpr int N = 256 * R3, NB = N / 16;
-  __shared__ __align__(16) V2<float> lds_all[WPB * N];
-  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int u = lane & 15, q = lane >> 4;
-  V2<float>* lds = lds_all + wave * N;
-  GlobalPtr<float> g;
-  init_global(g, a, (int64_t)blockIdx.x * WPB + wave);
-  g.stride = a.stride;
-  const bool sw = a.backward != 0;
+  const int lane = threadIdx.x & 63, u = lane & 15, q = lane >> 4;
+  const int64_t off = xform_offset(b, a), st = a.stride;
+  const V2<float>* src = static_cast<const V2<float>*>(a.src ? a.src : a.data) + off;
+  V2<float>* dst = static_cast<V2<float>*>(a.data) + off;
   const V2<float>* tw = static_cast<const V2<float>*>(a.ptw);
   const MfmaDft16<MODE> dft;
   V2<float> v[R3][4];
   // pass 1: butterfly 16 i + u, input m = 4 q + r at element 16 i + u + NB m
+  {
+    const V2<float>* ld = src + (int64_t)(u + 4 * NB * q) * st;
+    const int64_t s16 = 16 * st, sNB = NB * st;
 #pragma unroll
-  for (int i = 0; i < R3; ++i)
+    for (int i = 0; i < R3; ++i)
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      V2<float> x = mk<float>(0.f, 0.f);
-      if (g.valid) x = g.load(16 * i + u + NB * (4 * q + r));
-      v[i][r] = sw ? swp<float>(x) : x;
-    }
+      for (int r = 0; r < 4; ++r) {
+        const V2<float> x = ld[i * s16 + r * sNB];
+        v[i][r] = SW ? swp<float>(x) : x;
+      }
+  }
 #pragma unroll
   for (int i = 0; i < R3; ++i) {
     V2<float> y[4];
-    dft.apply(v[i], y);
+    dft.apply(v[i], y, split_sign(b, 16 * i + u));
     // output m of butterfly jj lands at 16 jj + m
 #pragma unroll
     for (int r = 0; r < 4; ++r) lds[split_row_slot<R3>(256 * i + 16 * u + 4 * q + r)] = y[r];
@@ -386,21 +418,21 @@ __global__ void __launch_bounds__(64 * WPB) c2c_split_row_kernel(StockhamArgs a)
   // pass 2: butterfly 16 i + u (span 16), input m = 4 q + r at 16 i + u + NB m
   V2<float> t2[4];
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
-    const int m = 4 * q + r;
-    t2[r] = m > 0 ? tw[m * 16 + u - 1] : mk<float>(1.f, 0.f);
-  }
+  for (int r = 0; r < 4; ++r) t2[r] = split_pass2_twiddle(tw, 4 * q + r, u);
 #pragma unroll
   for (int i = 0; i < R3; ++i) {
     V2<float> x[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r)
       x[r] = cmul<float>(lds[split_row_slot<R3>(16 * i + u + NB * (4 * q + r))], t2[r]);
-    dft.apply(x, v[i]);
+    dft.apply(x, v[i], split_sign(b, 16 * i + u));
   }
   // pass 3: butterfly jj = u + 16 m holds output m of pass-2 butterflies
-  // 16 i + u, i < R3, which land at 256 i + u + 16 m
+  // 16 i + u, i < R3, which land at 256 i + u + 16 m; outputs to HBM (a
+  // scale of 1 multiplies exactly)
   const float sc = (float)a.scale;
+  V2<float>* sp = dst + (int64_t)(u + 64 * q) * st;
+  const int64_t s16 = 16 * st, s256 = 256 * st;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
     const int jj = u + 16 * (4 * q + r);
@@ -410,124 +442,131 @@ __global__ void __launch_bounds__(64 * WPB) c2c_split_row_kernel(StockhamArgs a)
     split_last_pass<R3>(t, tw, jj);
 #pragma unroll
     for (int i = 0; i < R3; ++i) {
-      V2<float> x = t[i];
-      if (a.use_scale) x = scl<float>(x, sc);
-      if (sw) x = swp<float>(x);
-      if (g.valid) g.store(jj + 256 * i, x);
+      const V2<float> x = scl<float>(t[i], sc);
+      sp[r * s16 + i * s256] = SW ? swp<float>(x) : x;
     }
   }
+}
+
+template <int MODE, int R3, int WPB>
+__global__ void __launch_bounds__(64 * WPB) c2c_split_row_kernel(StockhamArgs a) {
+  __shared__ __align__(16) V2<float> lds_all[WPB * 256 * R3];
+  const int wave = threadIdx.x >> 6;
+  const int64_t b = (int64_t)blockIdx.x * WPB + wave;
+  if (b >= a.total) return;  // whole waves only; no workgroup barrier follows
+  V2<float>* lds = lds_all + wave * 256 * R3;
+  if (a.backward) split_row_body<MODE, R3, WPB, true>(a, lds, b);
+  else split_row_body<MODE, R3, WPB, false>(a, lds, b);
 }
 
 // Batch-contiguous layout: lane column u indexes 16 adjacent transforms (a
 // 128-byte run per element), W waves share a group of 16 transforms and CG
 // groups make a workgroup.  The exchange goes through an LDS tile of
-// N x 16 CG floats, one component at a time; element e of transform c sits
-// at e * C + c with the four 16-float quarters of every 64-float bank row
-// permuted by bits 2-3 and L-(L+1) of e, which keeps the pass-1 stores
-// (e = 16 jj + 4 q + r) and the pass-2 loads (e = jj + (N/16)(4 q + r)) free
-// of bank conflicts.
+// N x 16 CG complex values: element e of transform c sits at e * C + c with
+// bit 4 of that index (the half of a 64-bank row) XOR-ed by bit 2 and bit L
+// of e, which splits the lanes of both the pass-1 stores (e = 16 jj + 4 q +
+// r) and the pass-2 loads (e = jj + (N/16)(4 q + r)) evenly over the two
+// halves.
 template <int N, int C> struct SplitColLds {
   static constexpr int L = (N == 256 ? 6 : N == 512 ? 7 : N == 1024 ? 8 : 9);  // log2(N / 4)
   static constexpr int size = N * C;
   __device__ static __forceinline__ int at(int e, int c) {
-    const int key = ((e >> 2) ^ (e >> L)) & 3;
+    const int key = ((e >> 2) ^ (e >> L)) & 1;
     return (e * C + c) ^ (key << 4);
   }
 };
 
-template <int MODE, int R3, int W, int CG>
-__global__ void __launch_bounds__(64 * W * CG) c2c_split_col_kernel(StockhamArgs a) {
+template <int MODE, int R3, int W, int CG, bool SW>
+__device__ __forceinline__ void split_col_body(const StockhamArgs& a, V2<float>* lds) {
   constexpr int N = 256 * R3, NB = N / 16, C = 16 * CG;
   constexpr int P1 = NB / W;  // pass-1 butterflies per lane
   constexpr int P2 = 16 / W;  // pass-2 butterfly columns per lane
   static_assert(16 % W == 0 && P1 == P2 * R3, "waves must split the butterflies evenly");
   using L = SplitColLds<N, C>;
-  __shared__ __align__(16) float lds[L::size];
   const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
   const int u = lane & 15, q = lane >> 4;
   const int cg = wave % CG, w = wave / CG;
   const int c = 16 * cg + u;
   const int64_t tile = tile_order(blockIdx.x, gridDim.x, a.xcd_remap);
-  GlobalPtr<float> g;
-  init_global(g, a, tile * C + c);
-  g.stride = a.stride;
-  const bool sw = a.backward != 0;
+  const int64_t b = tile * C + c;
+  const bool valid = b < a.total;
+  // lanes past the end of the batch read the last transform and store nothing
+  const int64_t off = xform_offset(valid ? b : a.total - 1, a), st = a.stride;
+  const V2<float>* src = static_cast<const V2<float>*>(a.src ? a.src : a.data) + off;
+  V2<float>* dst = static_cast<V2<float>*>(a.data) + off;
   const V2<float>* tw = static_cast<const V2<float>*>(a.ptw);
   const MfmaDft16<MODE> dft;
   V2<float> v[P1][4];
   // pass 1: butterfly w + W i, input m = 4 q + r at element jj + NB m
-#pragma unroll
-  for (int i = 0; i < P1; ++i)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      V2<float> x = mk<float>(0.f, 0.f);
-      if (g.valid) x = g.load(w + W * i + NB * (4 * q + r));
-      v[i][r] = sw ? swp<float>(x) : x;
-    }
-#pragma unroll
-  for (int i = 0; i < P1; ++i) {
-    V2<float> y[4];
-    dft.apply(v[i], y);
-#pragma unroll
-    for (int r = 0; r < 4; ++r) v[i][r] = y[r];
-  }
-  // exchange: pass-1 output m of butterfly jj sits at 16 jj + m; pass 2
-  // takes butterflies b + 16 k2, b = w + W k, k2 < R3, at slot k R3 + k2
-#pragma unroll
-  for (int comp = 0; comp < 2; ++comp) {
+  {
+    const V2<float>* ld = src + (int64_t)(w + 4 * NB * q) * st;
+    const int64_t sW = W * st, sNB = NB * st;
 #pragma unroll
     for (int i = 0; i < P1; ++i)
 #pragma unroll
-      for (int r = 0; r < 4; ++r)
-        lds[L::at(16 * (w + W * i) + 4 * q + r, c)] = comp ? v[i][r].y : v[i][r].x;
-    __syncthreads();
-#pragma unroll
-    for (int k = 0; k < P2; ++k)
-#pragma unroll
-      for (int k2 = 0; k2 < R3; ++k2)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const float s = lds[L::at(w + W * k + 16 * k2 + NB * (4 * q + r), c)];
-          if (comp) v[k * R3 + k2][r].y = s;
-          else v[k * R3 + k2][r].x = s;
-        }
-    __syncthreads();
+      for (int r = 0; r < 4; ++r) {
+        const V2<float> x = ld[i * sW + r * sNB];
+        v[i][r] = SW ? swp<float>(x) : x;
+      }
   }
+#pragma unroll
+  for (int i = 0; i < P1; ++i) {
+    V2<float> y[4];
+    dft.apply(v[i], y, split_sign(b, w + W * i));
+    // output m of butterfly jj lands at 16 jj + m
+#pragma unroll
+    for (int r = 0; r < 4; ++r) lds[L::at(16 * (w + W * i) + 4 * q + r, c)] = y[r];
+  }
+  __syncthreads();
+  // pass 2 takes butterflies b + 16 k2, b = w + W k, k2 < R3, at slot k R3 + k2
+#pragma unroll
+  for (int k = 0; k < P2; ++k)
+#pragma unroll
+    for (int k2 = 0; k2 < R3; ++k2)
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        v[k * R3 + k2][r] = lds[L::at(w + W * k + 16 * k2 + NB * (4 * q + r), c)];
   const float sc = (float)a.scale;
+  V2<float>* sp = dst + (int64_t)(w + 64 * q) * st;
+  const int64_t sW = W * st, s16 = 16 * st, s256 = 256 * st;
 #pragma unroll
   for (int k = 0; k < P2; ++k) {
-    const int b = w + W * k;
-    // pass 2 (span 16): twiddle W_256^{b m}
+    const int bb = w + W * k;
+    // pass 2 (span 16): twiddle W_256^{bb m}
     V2<float> t2[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = 4 * q + r;
-      t2[r] = m > 0 ? tw[m * 16 + b - 1] : mk<float>(1.f, 0.f);
-    }
+    for (int r = 0; r < 4; ++r) t2[r] = split_pass2_twiddle(tw, 4 * q + r, bb);
 #pragma unroll
     for (int k2 = 0; k2 < R3; ++k2) {
       V2<float> x[4];
 #pragma unroll
       for (int r = 0; r < 4; ++r) x[r] = cmul<float>(v[k * R3 + k2][r], t2[r]);
-      dft.apply(x, v[k * R3 + k2]);
+      dft.apply(x, v[k * R3 + k2], split_sign(b, bb + 16 * k2));
     }
-    // pass 3 in registers, then HBM
+    // pass 3 in registers, then HBM at element jj + 256 k2
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int jj = b + 16 * (4 * q + r);
+      const int jj = bb + 16 * (4 * q + r);
       V2<float> t[R3];
 #pragma unroll
       for (int k2 = 0; k2 < R3; ++k2) t[k2] = v[k * R3 + k2][r];
       split_last_pass<R3>(t, tw, jj);
+      if (valid) {
 #pragma unroll
-      for (int k2 = 0; k2 < R3; ++k2) {
-        V2<float> x = t[k2];
-        if (a.use_scale) x = scl<float>(x, sc);
-        if (sw) x = swp<float>(x);
-        if (g.valid) g.store(jj + 256 * k2, x);
+        for (int k2 = 0; k2 < R3; ++k2) {
+          const V2<float> x = scl<float>(t[k2], sc);
+          sp[k * sW + r * s16 + k2 * s256] = SW ? swp<float>(x) : x;
+        }
       }
     }
   }
+}
+
+template <int MODE, int R3, int W, int CG>
+__global__ void __launch_bounds__(64 * W * CG) c2c_split_col_kernel(StockhamArgs a) {
+  __shared__ __align__(16) V2<float> lds[SplitColLds<256 * R3, 16 * CG>::size];
+  if (a.backward) split_col_body<MODE, R3, W, CG, true>(a, lds);
+  else split_col_body<MODE, R3, W, CG, false>(a, lds);
 }
 
 }  // namespace pfx
