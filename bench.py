@@ -10,7 +10,10 @@ dominant stage kernel (HIP events on the launch stream), and a CPU baseline
 (the FP64 oracle on a bounded sample of the same per-axis transforms).
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 1024] [--no-check]
-                  [--no-cpu] [--precision f32|f32_split_f16|f32_split_bf16]
+                  [--no-cpu] [--no-split] [--precision f32|f32_split_f16|f32_split_bf16]
+
+With FP32 precision the same line also carries ``split_vs_plain``: both
+split-precision MFMA forms planned, checked and timed on the same input.
 """
 from __future__ import annotations
 
@@ -282,6 +285,8 @@ def main():
                     help="batch workload: skip the one-ParaFaFT-object-per-shape baseline")
     ap.add_argument("--unfused", action="store_true",
                     help="pencil workload: separate pack / self copy / unpack launches")
+    ap.add_argument("--no-split", action="store_true",
+                    help="cube workload: skip the split-precision forms timed after plain FP32")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 4 GiB (the library's 2D-copy "
                          "kernel) five times, to scale rocprof FETCH/WRITE_SIZE counters")
@@ -372,25 +377,35 @@ def main():
                                  if chunk else "stage by stage"))
 
     # ---- accuracy (outside the timed region): FP32 vs FP64 on the same input
-    acc = {}
-    if not args.no_check and prec != A.PFX_F64:
+    def accuracy(ndh):
         log("[bench] accuracy check vs FP64 path")
         x64 = x.to(torch.complex128)
         nd64 = C.c_void_p()
         A.check(L.pfx_ndplan_create(C.byref(nd64), A.PFX_F64, A.PFX_C2C, 3, shape), "nd64")
         y = x.clone()
-        A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, y.data_ptr(), 1.0, sp), "fwd32")
+        A.check(L.pfx_ndplan_execute(ndh, A.PFX_FORWARD, y.data_ptr(), 1.0, sp), "fwd32")
         A.check(L.pfx_ndplan_execute(nd64.value, A.PFX_FORWARD, x64.data_ptr(), 1.0, sp), "fwd64")
         torch.cuda.synchronize()
         diff = (y.to(torch.complex128) - x64)
-        acc["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(diff) /
-                                      torch.linalg.vector_norm(x64))
-        acc["max_abs_vs_fp64"] = float(diff.abs().max())
-        acc["max_abs_rel_to_peak"] = acc["max_abs_vs_fp64"] / float(x64.abs().max())
+        r = {"rel_l2_vs_fp64": float(torch.linalg.vector_norm(diff) /
+                                     torch.linalg.vector_norm(x64)),
+             "max_abs_vs_fp64": float(diff.abs().max())}
+        r["max_abs_rel_to_peak"] = r["max_abs_vs_fp64"] / float(x64.abs().max())
         del diff, y, x64
         L.pfx_ndplan_destroy(nd64.value)
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
+        return r
+
+    def regenerate():
+        g.manual_seed(20241203)
+        xr = torch.view_as_real(x.view(-1))
+        xr.copy_(torch.rand(xr.shape, device=dev, generator=g, dtype=rdt) * 2 - 1)
+        torch.cuda.synchronize()
+
+    acc = {}
+    if not args.no_check and prec != A.PFX_F64:
+        acc = accuracy(nd.value)
 
     # timed parts: the D stages, or (chunked) axes 2+1 together and axis 0
     parts = [[h] for h in stages] if not chunk else [None, [stages[D - 1]]]
@@ -530,6 +545,45 @@ def main():
         "cpu_baseline": cpu,
     }
     L.pfx_ndplan_destroy(nd.value)
+
+    # ---- split-precision MFMA forms on the same grid and input (the metric's
+    # "split vs plain"): planned the same way, checked against the same FP64
+    # path, timed over the same K forwards
+    if prec == A.PFX_F32 and not args.no_split:
+        split = {}
+        for mode in ("f32_split_f16", "f32_split_bf16"):
+            nds = C.c_void_p()
+            A.check(L.pfx_ndplan_create(C.byref(nds), P.backend.precision_code(mode), A.PFX_C2C,
+                                        3, shape), "pfx_ndplan_create")
+            if flag != A.PFX_PLAN_ESTIMATE:
+                log(f"[bench] planning {mode} ({args.plan})")
+                A.check(L.pfx_ndplan_tune(nds.value, flag, x.data_ptr(), sp), "pfx_ndplan_tune")
+                torch.cuda.synchronize()
+                regenerate()
+            sacc = accuracy(nds.value) if not args.no_check else {}
+            for _ in range(args.warmup):
+                A.check(L.pfx_ndplan_execute(nds.value, A.PFX_FORWARD, x.data_ptr(), 1.0, sp), "fwd")
+            torch.cuda.synchronize()
+            s0 = time.perf_counter()
+            for _ in range(args.steps):
+                A.check(L.pfx_ndplan_execute(nds.value, A.PFX_FORWARD, x.data_ptr(), 1.0, sp), "fwd")
+            torch.cuda.synchronize()
+            sms = (time.perf_counter() - s0) * 1e3 / args.steps
+            schunk = C.c_int()
+            A.check(L.pfx_ndplan_chunk(nds.value, C.byref(schunk)), "pfx_ndplan_chunk")
+            kern = []
+            for st in range(D):
+                h = C.c_void_p()
+                A.check(L.pfx_ndplan_stage(nds, st, C.byref(h)), "pfx_ndplan_stage")
+                kern.append(describe(h.value))
+            split[mode] = {"ms_per_step": round(sms, 4),
+                           "gflops": round(flops / (sms * 1e-3) / 1e9, 2),
+                           "speed_vs_plain": round(ms_per_step / sms, 4),
+                           "accuracy": sacc, "chunk_planes": schunk.value,
+                           "stage_kernels": kern}
+            log(f"[bench] {mode}: {sms:.3f} ms/step ({ms_per_step / sms:.3f}x plain)")
+            L.pfx_ndplan_destroy(nds.value)
+        out["split_vs_plain"] = split
     print(json.dumps(out), flush=True)
 
 
