@@ -285,6 +285,9 @@ def main():
                     help="batch workload: skip the one-ParaFaFT-object-per-shape baseline")
     ap.add_argument("--unfused", action="store_true",
                     help="pencil workload: separate pack / self copy / unpack launches")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="cube workload: skip the HIP-graph replay (profiling runs keep only "
+                         "the timed region's dispatches in the trace)")
     ap.add_argument("--no-split", action="store_true",
                     help="cube workload: skip the split-precision forms timed after plain FP32")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -455,6 +458,8 @@ def main():
     # timed region above; reported beside it)
     graph_ms = None
     try:
+        if args.no_graph:
+            raise RuntimeError("skipped (--no-graph)")
         gstream = torch.cuda.Stream()      # the legacy null stream cannot be captured
         gsp = gstream.cuda_stream
         gh = C.c_void_p()
