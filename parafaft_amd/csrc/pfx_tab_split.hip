@@ -35,7 +35,7 @@ template <int R3> constexpr Radices split_radices() {
   {MODE, KIND_C2C, 256 * R3, 0, 64, WPB, WPB * 256 * R3 * 8, 2, "split-reg:16, 16, " #R3,  \
    &launch_split_row<MODE, R3, WPB>, split_radices<R3>(), nullptr},
 #define PFX_SPLIT_COL(MODE, R3, W, CG)                                                      \
-  {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, 256 * R3 * 16 * CG * 4, 2,                  \
+  {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, 256 * R3 * 16 * CG * 8, 2,                  \
    "split-reg:16, 16, " #R3 "/w" #W, &launch_split_col<MODE, R3, W, CG>, split_radices<R3>(), \
    nullptr},
 #define PFX_SPLIT_REG(MODE)                                                                 \
