@@ -186,7 +186,53 @@ __device__ __forceinline__ void stockham_pass(int j, const V2<T>* __restrict__ s
 // HBM side of one transform.  SEG = false is the plain strided row/column;
 // SEG = true adds the segment maps of the fused exchange (a separate kernel
 // instance, so the plain kernels carry none of its addressing).
-template <typename T, bool SEG = false> struct GlobalPtr {
+// Segment lookup tables in LDS, built once per workgroup by the segmented
+// C2C kernels that have room for them: entry e of the split axis holds
+// (p << 12) | (e - s[p]) (N <= 4096), and each part its segment address and
+// block count.  An access is then two LDS reads and one multiply-add, with
+// no per-access scalar search between a lane's loads.
+struct SegPart {
+  void* seg;
+  int n;
+  int pad;
+};
+template <int N> struct SegTabs {
+  static_assert(N <= 4096, "table offsets are 12 bits");
+  static constexpr int part_bytes = kMaxSegPeers * (int)sizeof(SegPart);
+  static constexpr int bytes = 2 * part_bytes + 2 * ((N * 2 + 15) / 16 * 16);
+  __device__ static __forceinline__ SegPart* parts(unsigned char* m, int side) {
+    return reinterpret_cast<SegPart*>(m + side * part_bytes);
+  }
+  __device__ static __forceinline__ uint16_t* ent(unsigned char* m, int side) {
+    return reinterpret_cast<uint16_t*>(m + 2 * part_bytes + side * ((N * 2 + 15) / 16 * 16));
+  }
+  // all threads of the workgroup take part; a barrier must follow
+  __device__ static __forceinline__ void fill(unsigned char* m, int side, KSegMap* map) {
+    SegPart* pp = parts(m, side);
+    for (int q = 0; q < map->nparts; ++q)
+      if ((int)threadIdx.x == q) {
+        SegPart sp;
+        sp.seg = map->seg[q];
+        sp.n = map->n[q];
+        sp.pad = 0;
+        pp[q] = sp;
+      }
+    uint16_t* en = ent(m, side);
+    for (int e = threadIdx.x; e < N; e += blockDim.x) {
+      int p = 0, s0 = 0;
+      for (int q = 1; q < map->nparts; ++q) {
+        const int sq = map->s[q];
+        if (e >= sq) {
+          p = q;
+          s0 = sq;
+        }
+      }
+      en[e] = (uint16_t)((p << 12) | (e - s0));
+    }
+  }
+};
+
+template <typename T, bool SEG = false, bool TAB = false> struct GlobalPtr {
   V2<T>* p;
   const V2<T>* pin;          // load side (== p in place)
   int64_t stride;
@@ -198,7 +244,7 @@ template <typename T, bool SEG = false> struct GlobalPtr {
   __device__ __forceinline__ void store(int e, V2<T> v) const { p[(int64_t)e * stride] = v; }
 };
 
-template <typename T> struct GlobalPtr<T, true> {
+template <typename T, bool TAB> struct GlobalPtr<T, true, TAB> {
   V2<T>* p;
   const V2<T>* pin;
   int64_t stride;
@@ -212,19 +258,78 @@ template <typename T> struct GlobalPtr<T, true> {
   int64_t in_trail, out_trail;
   int lane_j;                // this lane's position among its transform's threads
   int blk;                   // threads per transform
+  // TAB: the maps' LDS tables (set by the kernel body after building them)
+  const uint16_t* in_ent;
+  const uint16_t* out_ent;
+  const SegPart* in_parts;
+  const SegPart* out_parts;
+  int64_t in_trailing, out_trailing;
   // first element of the wave's block (e - lane_j is the same on every lane)
   __device__ __forceinline__ int block_start(int e) const {
     return __builtin_amdgcn_readfirstlane(e - lane_j);
   }
+  // segment addresses are device memory: accessed as global (not generic,
+  // flat) two-element vectors, one load or store instruction per element
+  typedef T E2 __attribute__((ext_vector_type(2)));
+  using GV = __attribute__((address_space(1))) E2;
+  __device__ __forceinline__ static GV* tab_addr(const uint16_t* ent, const SegPart* parts,
+                                                 uint32_t lead, int64_t trail,
+                                                 int64_t trailing, int e) {
+    const uint32_t t = ent[e];
+    const SegPart sp = parts[t >> 12];
+    const uint32_t row = lead * (uint32_t)sp.n + (t & 0xFFFu);
+    return (GV*)sp.seg + ((uint64_t)row * (uint64_t)trailing + trail);
+  }
   __device__ __forceinline__ V2<T> load(int e) const {
-    if (in_map) return *seg_addr<T>(in_map, in_lead, in_trail, e, block_start(e), blk);
+    if (in_map) {
+      if constexpr (TAB) {
+        const E2 x = *tab_addr(in_ent, in_parts, in_lead, in_trail, in_trailing, e);
+        return mk<T>(x.x, x.y);
+      } else {
+        return *seg_addr<T>(in_map, in_lead, in_trail, e, block_start(e), blk);
+      }
+    }
     return pin[(int64_t)e * stride];
   }
   __device__ __forceinline__ void store(int e, V2<T> v) const {
-    if (out_map) *seg_addr<T>(out_map, out_lead, out_trail, e, block_start(e), blk) = v;
-    else p[(int64_t)e * stride] = v;
+    if (out_map) {
+      if constexpr (TAB) {
+        E2 x;
+        x.x = v.x;
+        x.y = v.y;
+        *tab_addr(out_ent, out_parts, out_lead, out_trail, out_trailing, e) = x;
+      } else {
+        *seg_addr<T>(out_map, out_lead, out_trail, e, block_start(e), blk) = v;
+      }
+    } else {
+      p[(int64_t)e * stride] = v;
+    }
   }
 };
+
+// Build the LDS tables of the maps a segmented launch uses and point g at
+// them.  Every thread of the workgroup calls this (it ends in a barrier).
+template <typename T, int N>
+__device__ __forceinline__ void seg_tables(GlobalPtr<T, true, true>& g, unsigned char* mem) {
+  using ST = SegTabs<N>;
+  if (g.in_map) ST::fill(mem, 0, g.in_map);
+  if (g.out_map) ST::fill(mem, 1, g.out_map);
+  __syncthreads();
+  g.in_ent = ST::ent(mem, 0);
+  g.out_ent = ST::ent(mem, 1);
+  g.in_parts = ST::parts(mem, 0);
+  g.out_parts = ST::parts(mem, 1);
+  g.in_trailing = g.in_map ? g.in_map->trailing : 0;
+  g.out_trailing = g.out_map ? g.out_map->trailing : 0;
+}
+
+// LDS budget of one workgroup; a segmented kernel builds the tables when its
+// own block leaves room for them.
+constexpr int kLdsMax = 163840;
+template <bool SEG, int N, int BYTES> constexpr bool seg_tab_fits() {
+  if constexpr (!SEG || N > 4096) return false;
+  else return BYTES + SegTabs<N>::bytes <= kLdsMax;
+}
 
 __device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a) {
   const int64_t o = b / a.inner;
@@ -232,8 +337,8 @@ __device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a
   return o * a.dist_outer + i * a.dist;
 }
 
-template <typename T, bool SEG>
-__device__ __forceinline__ void init_global(GlobalPtr<T, SEG>& g, const StockhamArgs& a, int64_t b) {
+template <typename T, bool SEG, bool TAB>
+__device__ __forceinline__ void init_global(GlobalPtr<T, SEG, TAB>& g, const StockhamArgs& a, int64_t b) {
   g.valid = b < a.total;
   const V2<T>* in_base = static_cast<const V2<T>*>(a.src ? a.src : a.data);
   if constexpr (!SEG) {
@@ -370,7 +475,8 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, COL>;
   using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
-  __shared__ __align__(16) unsigned char smem[B::bytes];
+  constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
+  __shared__ __align__(16) unsigned char smem[B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0)];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
   const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
@@ -378,12 +484,13 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
   thread_coords<TPT, C, COL>(j, c);
   if (COL) tile = tile_order(tile, ntiles, a.xcd_remap);
   const int64_t b = tile * C + c;
-  GlobalPtr<T, SEG> g;
+  GlobalPtr<T, SEG, TAB> g;
   init_global(g, a, b);
   if constexpr (SEG) {
     g.lane_j = j;
     g.blk = TPT;
   }
+  if constexpr (TAB) seg_tables<T, N>(g, smem + B::bytes);
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
@@ -652,7 +759,8 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
   constexpr int NB0 = N / R0;
   using WL = WideLayout<N, C, ROWL>;
   using B = LdsBlock<T, T, (!ROWL && sizeof...(Rs) > 1 ? N : 0), WL::size>;
-  __shared__ __align__(16) unsigned char smem[B::bytes];
+  constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
+  __shared__ __align__(16) unsigned char smem[B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0)];
   T* lds = B::data(smem);
   const V2<T>* tw;
   int c, j;
@@ -667,12 +775,13 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
     tile = tile_order(tile, ntiles, a.xcd_remap);
   }
   const int64_t b = tile * C + c;
-  GlobalPtr<T, SEG> g;
+  GlobalPtr<T, SEG, TAB> g;
   init_global(g, a, b);
   if constexpr (SEG) {
     g.lane_j = j;
     g.blk = TPT;
   }
+  if constexpr (TAB) seg_tables<T, N>(g, smem + B::bytes);
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
