@@ -121,14 +121,16 @@ template <> struct MfmaDft16<SPLIT_F16> {
     const uint32_t sbit = neg ? 0x80000000u : 0u;
     const float sin_ = __uint_as_float(((uint32_t)(127 - sh) << 23) | sbit);
     const float sout = __uint_as_float(((uint32_t)(127 + sh) << 23) | sbit);
+    // hi = fp16(x s); lo = fp16(x s - hi), with x s - hi exact (s is a power
+    // of two) and formed by one mixed-precision fma (v_fma_mixlo_f16)
     half4_t xrh, xrl, xih, xil;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const float xr = x[r].x * sin_, xi = x[r].y * sin_;
-      xrh[r] = (_Float16)xr;
-      xrl[r] = (_Float16)(xr - (float)xrh[r]);
-      xih[r] = (_Float16)xi;
-      xil[r] = (_Float16)(xi - (float)xih[r]);
+      const _Float16 hr = (_Float16)(x[r].x * sin_), hi = (_Float16)(x[r].y * sin_);
+      xrh[r] = hr;
+      xrl[r] = (_Float16)__builtin_fmaf(x[r].x, sin_, -(float)hr);
+      xih[r] = hi;
+      xil[r] = (_Float16)__builtin_fmaf(x[r].y, sin_, -(float)hi);
     }
     float4_t zr = {0.f, 0.f, 0.f, 0.f}, zi = zr;
     // small products first, then the leading ones
