@@ -571,4 +571,116 @@ __global__ void __launch_bounds__(64 * W * CG) c2c_split_col_kernel(StockhamArgs
   else split_col_body<MODE, R3, W, CG, false>(a, lds);
 }
 
+// Batch-contiguous layout with the exchange in R3 rounds.  Input m of
+// pass-2 butterfly b + 16 k2 is output b of pass-1 butterfly R3 m + k2, so
+// round k2 runs the sixteen pass-1 butterflies R3 t + k2, exchanges their
+// outputs as a 16 x 16 transpose per transform (slot (t, m') written, slot
+// (m, b) read), and runs the sixteen pass-2 butterflies b + 16 k2.  Two
+// alternating buffers of 256 x 16 CG complex values stand in for the whole
+// N x 16 CG tile (64 KB instead of 256 KB at N = 2048, 16 transforms), which
+// gives N = 2048 a register-resident batch-contiguous kernel; one barrier
+// per round (a buffer is rewritten two rounds later, after every lane has
+// passed the barrier between).  At N = 1024 the smaller LDS would let two
+// 512-thread workgroups share a CU, but such a kernel needs 150 VGPRs and
+// spills when held to 128, and was measured no faster.  Slot (t, m)
+// has bit 4 of its index (the half of a 64-bank row) XOR-ed by bit 2 of t
+// and of m, which spreads the four lane groups of both the stores (m' =
+// 4 q + r) and the loads (t = 4 q + r) over both halves.
+template <int C> __device__ __forceinline__ int split_round_slot(int t, int m, int c) {
+  const int key = ((m >> 2) ^ (t >> 2)) & 1;
+  return ((t * 16 + m) * C + c) ^ (key << 4);
+}
+
+template <int MODE, int R3, int W, int CG, bool SW>
+__device__ __forceinline__ void split_colr_body(const StockhamArgs& a, V2<float>* lds) {
+  constexpr int N = 256 * R3, NB = N / 16, C = 16 * CG;
+  constexpr int G = 16 / W;  // butterflies per lane and round, in each pass
+  static_assert(16 % W == 0, "waves must split the sixteen butterflies of a round");
+  const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int u = lane & 15, q = lane >> 4;
+  const int cg = wave % CG, w = wave / CG;
+  const int c = 16 * cg + u;
+  const int64_t tile = tile_order(blockIdx.x, gridDim.x, a.xcd_remap);
+  const int64_t b = tile * C + c;
+  const bool valid = b < a.total;
+  // lanes past the end of the batch read the last transform and store nothing
+  const int64_t off = xform_offset(valid ? b : a.total - 1, a), st = a.stride;
+  const V2<float>* src = static_cast<const V2<float>*>(a.src ? a.src : a.data) + off;
+  V2<float>* dst = static_cast<V2<float>*>(a.data) + off;
+  const V2<float>* tw = static_cast<const V2<float>*>(a.ptw);
+  const MfmaDft16<MODE> dft;
+  V2<float> v[R3 * G][4];
+  // every pass-1 input up front: slot k2 G + i holds butterfly R3 t + k2,
+  // t = w + W i, inputs m = 4 q + r at element R3 t + k2 + NB m
+  {
+    const V2<float>* ld = src + (int64_t)(R3 * w + 4 * NB * q) * st;
+    const int64_t sT = R3 * W * st, sNB = NB * st;
+#pragma unroll
+    for (int k2 = 0; k2 < R3; ++k2)
+#pragma unroll
+      for (int i = 0; i < G; ++i)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const V2<float> x = ld[k2 * st + i * sT + r * sNB];
+          v[k2 * G + i][r] = SW ? swp<float>(x) : x;
+        }
+  }
+#pragma unroll
+  for (int k2 = 0; k2 < R3; ++k2) {
+    V2<float>* buf = lds + (k2 & 1) * 256 * C;
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int t = w + W * i;
+      V2<float> y[4];
+      dft.apply(v[k2 * G + i], y, split_sign(b, R3 * t + k2));
+#pragma unroll
+      for (int r = 0; r < 4; ++r) buf[split_round_slot<C>(t, 4 * q + r, c)] = y[r];
+    }
+    __syncthreads();
+    // pass 2 (span 16): butterfly bb + 16 k2, input m from slot (m, bb),
+    // twiddle W_256^{bb m}
+#pragma unroll
+    for (int i = 0; i < G; ++i) {
+      const int bb = w + W * i;
+      V2<float> x[4];
+#pragma unroll
+      for (int r = 0; r < 4; ++r)
+        x[r] = cmul<float>(buf[split_round_slot<C>(4 * q + r, bb, c)],
+                           split_pass2_twiddle(tw, 4 * q + r, bb));
+      dft.apply(x, v[k2 * G + i], split_sign(b, bb + 16 * k2));
+    }
+  }
+  // pass 3 in registers: butterfly jj = bb + 16 m takes output m of pass-2
+  // butterflies bb + 16 k2 (slots k2 G + i); output k2 lands at jj + 256 k2
+  const float sc = (float)a.scale;
+  V2<float>* sp = dst + (int64_t)(w + 64 * q) * st;
+  const int64_t sW = W * st, s16 = 16 * st, s256 = 256 * st;
+#pragma unroll
+  for (int i = 0; i < G; ++i) {
+    const int bb = w + W * i;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int jj = bb + 16 * (4 * q + r);
+      V2<float> t[R3];
+#pragma unroll
+      for (int k2 = 0; k2 < R3; ++k2) t[k2] = v[k2 * G + i][r];
+      split_last_pass<R3>(t, tw, jj);
+      if (valid) {
+#pragma unroll
+        for (int k2 = 0; k2 < R3; ++k2) {
+          const V2<float> x = scl<float>(t[k2], sc);
+          sp[i * sW + r * s16 + k2 * s256] = SW ? swp<float>(x) : x;
+        }
+      }
+    }
+  }
+}
+
+template <int MODE, int R3, int W, int CG>
+__global__ void __launch_bounds__(64 * W * CG) c2c_split_colr_kernel(StockhamArgs a) {
+  __shared__ __align__(16) V2<float> lds[(R3 > 1 ? 2 : 1) * 256 * 16 * CG];
+  if (a.backward) split_colr_body<MODE, R3, W, CG, true>(a, lds);
+  else split_colr_body<MODE, R3, W, CG, false>(a, lds);
+}
+
 }  // namespace pfx

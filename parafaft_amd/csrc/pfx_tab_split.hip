@@ -22,6 +22,11 @@ void launch_split_col(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + 16 * CG - 1) / (16 * CG);
   c2c_split_col_kernel<MODE, R3, W, CG><<<dim3((unsigned)nb), dim3(64 * W * CG), 0, s>>>(a);
 }
+template <int MODE, int R3, int W, int CG>
+void launch_split_colr(const StockhamArgs &a, hipStream_t s) {
+  const int64_t nb = (a.total + 16 * CG - 1) / (16 * CG);
+  c2c_split_colr_kernel<MODE, R3, W, CG><<<dim3((unsigned)nb), dim3(64 * W * CG), 0, s>>>(a);
+}
 template <int R3> constexpr Radices split_radices() {
   return R3 == 1 ? radices<16, 16>() : radices<16, 16, R3>();
 }
@@ -29,8 +34,8 @@ template <int R3> constexpr Radices split_radices() {
 // Register-resident kernels (passes 16, 16, R3): one wave per row transform
 // (WPB per workgroup), or 16 CG batch-contiguous transforms over W waves.
 // Batch-contiguous variants holding 32 values per lane at 1024 threads
-// (R3 = 4 with W = 8, CG = 2, and R3 = 8) exceed 128 registers and spill, so
-// they are not listed.
+// (R3 = 4 with W = 8, CG = 2) exceed 128 registers and spill, so they are
+// not listed; N = 2048 uses the round-exchange kernel.
 #define PFX_SPLIT_ROW(MODE, R3, WPB)                                                        \
   {MODE, KIND_C2C, 256 * R3, 0, 64, WPB, WPB * 256 * R3 * 8, 2, "split-reg:16, 16, " #R3,  \
    &launch_split_row<MODE, R3, WPB>, split_radices<R3>(), nullptr},
@@ -38,11 +43,16 @@ template <int R3> constexpr Radices split_radices() {
   {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, 256 * R3 * 16 * CG * 8, 2,                  \
    "split-reg:16, 16, " #R3 "/w" #W, &launch_split_col<MODE, R3, W, CG>, split_radices<R3>(), \
    nullptr},
+// ... exchanging in R3 rounds through two 256 x 16 CG buffers (N = 2048)
+#define PFX_SPLIT_COLR(MODE, R3, W, CG)                                                     \
+  {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, (R3 > 1 ? 2 : 1) * 256 * 16 * CG * 8, 2,     \
+   "split-reg:16, 16, " #R3 "/w" #W "/rounds", &launch_split_colr<MODE, R3, W, CG>,          \
+   split_radices<R3>(), nullptr},
 #define PFX_SPLIT_REG(MODE)                                                                 \
   PFX_SPLIT_ROW(MODE, 1, 4) PFX_SPLIT_ROW(MODE, 2, 4) PFX_SPLIT_ROW(MODE, 4, 4)             \
   PFX_SPLIT_ROW(MODE, 4, 2) PFX_SPLIT_ROW(MODE, 8, 4)                                       \
   PFX_SPLIT_COL(MODE, 1, 4, 2) PFX_SPLIT_COL(MODE, 2, 8, 2) PFX_SPLIT_COL(MODE, 4, 16, 1)   \
-  PFX_SPLIT_COL(MODE, 4, 8, 1)
+  PFX_SPLIT_COL(MODE, 4, 8, 1) PFX_SPLIT_COLR(MODE, 8, 16, 1)
 
 #define PFX_SPLIT_ENTRY(MODE, COL, TILE, TPT, ...)                                         \
   {MODE, KIND_C2C, prod<__VA_ARGS__>(), COL, TPT, TILE,                                     \

@@ -93,9 +93,8 @@ def test_split_every_candidate(mode, n, layout):
             O.c2c_batched(ref, n, batch, stride, dist, sign)
             err = rel_l2(to_host(d), ref * scale)
             assert err <= tol("f32", n), (name, direction, err)
-    # register-resident kernels exist for every case but 2048 batch-contiguous
-    # (32 values per lane at 1024 threads would spill)
-    assert any("split-reg" in nm for nm in names) == (layout == "row" or n < 2048), names
+    # a register-resident kernel is among the candidates for every case
+    assert any("split-reg" in nm for nm in names), names
 
 
 @pytest.mark.parametrize("mode", MODES)
