@@ -31,9 +31,9 @@ using parafaft::FFTPlanFlag;
 
 enum class Mode { Faithful, Fast };
 
-template <typename B> constexpr int nd_precision() {
-  return std::is_same<typename B::FloatType, double>::value ? PFX_F64 : PFX_F32;
-}
+// The backend's own precision code, the split-precision MFMA forms included
+// (MI355Backend<float, PFX_F32_SPLIT_F16> runs its N-D plan split too).
+template <typename B> constexpr int nd_precision() { return B::precision; }
 
 /// Measure / Patient planning of a one-rank N-D plan: the FFTW_MEASURE
 /// analogue for the fast path (kernels, and chunked or stage-by-stage

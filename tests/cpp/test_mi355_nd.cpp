@@ -5,7 +5,9 @@
 // 1e-10 for double, test_fftw_c2c_nd.cpp:119; < 1e-4 * peak for float,
 // test_fftw_c2c_nd_float.cpp:101-107), then backward with 1/N^D scaling
 // against the input (test_fftw_c2c_roundtrip_nd.cpp:73-93).  Both the
-// faithful driver and the one-GPU fast path run.  Returns the failure count.
+// faithful driver and the one-GPU fast path run, and the split-precision
+// MFMA backends (MI355Backend<float, PFX_F32_SPLIT_F16 / _BF16>) on lengths
+// that have radix-16 passes.  Returns the failure count.
 #include <cmath>
 #include <complex>
 #include <cstdio>
@@ -32,7 +34,13 @@ static void sync_copy(void *dst, const void *src, size_t bytes) {
   parafaft::check_pfx(pfx_stream_synchronize(nullptr), "sync");
 }
 
-template <typename T, int D>
+template <typename T> constexpr int plain_mode() { return sizeof(T) == 8 ? PFX_F64 : PFX_F32; }
+static const char *prec_name(int sm) {
+  return sm == PFX_F64 ? "double" : sm == PFX_F32 ? "float" : sm == PFX_F32_SPLIT_F16 ? "spl-f16"
+                                                                                     : "spl-bf16";
+}
+
+template <typename T, int D, int SM = plain_mode<T>()>
 static void c2c_case(int N, int shape_id, Mode mode,
                      parafaft::FFTPlanFlag flag = parafaft::FFTPlanFlag::Estimate) {
   int shape[D];
@@ -49,7 +57,7 @@ static void c2c_case(int N, int shape_id, Mode mode,
   double peak = 0;
   for (auto &v : ref) peak = std::max(peak, std::abs(v));
 
-  using B = parafaft::MI355Backend<T>;
+  using B = parafaft::MI355Backend<T, SM>;
   parafaft_mi355::ParaFaFT_C2C<D, B> fft(shape, flag, mode);
   typename B::ComplexBuffer d(fft.get_required_output_size());
   std::vector<std::complex<T>> h(tot);
@@ -65,10 +73,10 @@ static void c2c_case(int N, int shape_id, Mode mode,
   for (size_t i = 0; i < tot; ++i) e2 = std::max(e2, std::abs(std::complex<double>(h[i]) - x[i]));
   const bool dbl = sizeof(T) == 8;
   const bool ok = dbl ? (e1 < 1e-10 && e2 < 1e-10) : (e1 < 1e-4 * peak && e2 < 1e-4);
-  report(ok, "C2C", dbl ? "double" : "float", D, N, shape_id, mode, e1, e2);
+  report(ok, "C2C", prec_name(SM), D, N, shape_id, mode, e1, e2);
 }
 
-template <typename T, int D>
+template <typename T, int D, int SM = plain_mode<T>()>
 static void r2c_case(int N, int shape_id, Mode mode,
                      parafaft::FFTPlanFlag flag = parafaft::FFTPlanFlag::Estimate) {
   int shape[D];
@@ -89,7 +97,7 @@ static void r2c_case(int N, int shape_id, Mode mode,
   double peak = 0;
   for (double v : ref) peak = std::max(peak, std::abs(v));
 
-  using B = parafaft::MI355Backend<T>;
+  using B = parafaft::MI355Backend<T, SM>;
   parafaft_mi355::ParaFaFT_R2C<D, B> fft(shape, flag, mode);
   typename B::Buffer d(fft.get_required_output_size());
   sync_copy(d.data(), pad.data(), pad.size() * sizeof(T));
@@ -105,7 +113,7 @@ static void r2c_case(int N, int shape_id, Mode mode,
     for (int i = 0; i < N; ++i) e2 = std::max(e2, std::abs(double(out[r * pd + i]) - x[r * N + i]));
   const bool dbl = sizeof(T) == 8;
   const bool ok = dbl ? (e1 < 1e-10 && e2 < 1e-10) : (e1 < 1e-4 * peak && e2 < 1e-4);
-  report(ok, "R2C", dbl ? "double" : "float", D, N, shape_id, mode, e1, e2);
+  report(ok, "R2C", prec_name(SM), D, N, shape_id, mode, e1, e2);
 }
 
 // Fast mode with Measure planning (temporary planning buffer, kernels and
@@ -134,10 +142,58 @@ template <typename T> static void matrix() {
     }
 }
 
+template <class B> struct Tag { using type = B; };
+
+// The split backend really runs its own kernels on both driver paths: its
+// forward differs bit-wise from the plain float backend's on the same input.
+template <int SM, Mode M> static void split_is_not_plain() {
+  constexpr int D = 3, N = 256;
+  int shape[D] = {N, N, N};
+  const size_t tot = size_t(N) * N * N;
+  std::vector<double> x(tot);
+  orc_generate_shape(x.data(), D, N, 2);
+  std::vector<std::complex<float>> h(tot), r[2];
+  for (size_t i = 0; i < tot; ++i) h[i] = std::complex<float>(float(x[i]), 0.f);
+  auto run = [&](auto backend_tag, std::vector<std::complex<float>> &out) {
+    using B = typename decltype(backend_tag)::type;
+    parafaft_mi355::ParaFaFT_C2C<D, B> fft(shape, parafaft::FFTPlanFlag::Estimate, M);
+    typename B::ComplexBuffer d(fft.get_required_output_size());
+    sync_copy(d.data(), h.data(), tot * sizeof(h[0]));
+    fft.forward(d.data());
+    out.resize(tot);
+    sync_copy(out.data(), d.data(), tot * sizeof(h[0]));
+  };
+  run(Tag<parafaft::MI355Backend<float, SM>>{}, r[0]);
+  run(Tag<parafaft::MI355Backend<float>>{}, r[1]);
+  size_t diff = 0;
+  for (size_t i = 0; i < tot; ++i) diff += r[0][i] != r[1][i];
+  const bool ok = diff > 0;
+  std::printf("  C2C %s D3 N256 %s differs from plain float in %zu of %zu elements %s\n",
+              prec_name(SM), M == Mode::Fast ? "fast" : "faithful", diff, tot,
+              ok ? "[PASS]" : "[FAIL]");
+  if (!ok) ++failures;
+}
+
+// Split-precision backends: 256-point axes run their radix-16 passes on the
+// matrix cores (the R2C stage itself stays plain FP32); same tolerances as
+// float.
+template <int SM> static void split() {
+  split_is_not_plain<SM, Mode::Faithful>();
+  split_is_not_plain<SM, Mode::Fast>();
+  for (Mode m : {Mode::Faithful, Mode::Fast}) {
+    c2c_case<float, 2, SM>(256, 0, m);
+    c2c_case<float, 3, SM>(256, 2, m);
+    r2c_case<float, 3, SM>(256, 1, m);
+  }
+  c2c_case<float, 3, SM>(256, 1, Mode::Fast, parafaft::FFTPlanFlag::Measure);
+}
+
 int main() {
   try {
     matrix<double>();
     matrix<float>();
+    split<PFX_F32_SPLIT_F16>();
+    split<PFX_F32_SPLIT_BF16>();
   } catch (const std::exception &e) {
     std::printf("exception: %s\n", e.what());
     return 100;
