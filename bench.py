@@ -452,7 +452,12 @@ def main():
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     part_ms = [sum(ev[k][s][0].elapsed_time(ev[k][s][1]) for k in range(args.steps)) / args.steps
                for s in range(NP)]
-    log(f"[bench] ms/step {ms_per_step:.3f}, part ms {['%.3f' % t for t in part_ms]}")
+    # per-step event time, first part's start to last part's end, as a median
+    step_ev = sorted(ev[k][0][0].elapsed_time(ev[k][NP - 1][1]) for k in range(args.steps))
+    median_ms = step_ev[len(step_ev) // 2] if len(step_ev) % 2 else \
+        0.5 * (step_ev[len(step_ev) // 2 - 1] + step_ev[len(step_ev) // 2])
+    log(f"[bench] ms/step {ms_per_step:.3f} (median of events {median_ms:.3f}), "
+        f"part ms {['%.3f' % t for t in part_ms]}")
 
     # the same forward captured once as a HIP graph and replayed (outside the
     # timed region above; reported beside it)
@@ -480,6 +485,23 @@ def main():
         log(f"[bench] graph replay ms/step {graph_ms:.3f}")
     except Exception as e:  # graphs are reported, not required
         log(f"[bench] graph replay unavailable: {e}")
+
+    # forward then backward with the 1/N scale fused into the last pass (the
+    # reference benchmark's timed pair, bench_c2c.cpp:61-76), K pairs
+    fb_ms = None
+    if not args.no_graph:
+        inv = 1.0 / N
+        for _ in range(args.warmup):
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, x.data_ptr(), 1.0, sp), "fwd")
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_BACKWARD, x.data_ptr(), inv, sp), "bwd")
+        torch.cuda.synchronize()
+        f0 = time.perf_counter()
+        for _ in range(args.steps):
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, x.data_ptr(), 1.0, sp), "fwd")
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_BACKWARD, x.data_ptr(), inv, sp), "bwd")
+        torch.cuda.synchronize()
+        fb_ms = round((time.perf_counter() - f0) * 1e3 / args.steps, 4)
+        log(f"[bench] forward+backward ms {fb_ms}")
 
     flops = 5.0 * N * math.log2(N)
     gflops = flops / (ms_per_step * 1e-3) / 1e9
@@ -539,7 +561,9 @@ def main():
                    "stage_kernels": names,
                    "chunk_kernels": [describe(h) for h in cstages]},
         "hbm_tbs_algorithmic": round(tbs, 3),
+        "event_median_ms_per_step": round(median_ms, 4),
         "graph_replay_ms_per_step": round(graph_ms, 4) if graph_ms else None,
+        "forward_backward_ms": fb_ms,
         "part_ms": ({"axes 2+1 (chunked)": round(part_ms[0], 4), "axis 0": round(part_ms[1], 4)}
                     if chunk else {f"axis {D - 1 - s}": round(t, 4) for s, t in enumerate(part_ms)}),
         "accuracy": acc,
