@@ -290,9 +290,55 @@ public:
     backend_.sync();
   }
 
+  /// parafaft_r2c.hpp:804-823: contiguous real rows into padded rows of
+  /// 2(N/2+1) scalars, one 2D copy.
+  void copy_real_to_padded(const FloatType *real_input, FloatType *padded_output) const {
+    const std::size_t n = static_cast<std::size_t>(real_[D - 1]);
+    const std::size_t pd = 2 * static_cast<std::size_t>(cplx_[D - 1]);
+    backend_.memcpy2d(padded_output, pd * sizeof(FloatType), real_input, n * sizeof(FloatType),
+                      n * sizeof(FloatType), rprod(0, D - 1));
+  }
+  /// parafaft_r2c.hpp:825-847: padded rows back to contiguous real rows.
+  void copy_padded_to_real(const FloatType *padded_input, FloatType *real_output) const {
+    const std::size_t n = static_cast<std::size_t>(real_[D - 1]);
+    const std::size_t pd = 2 * static_cast<std::size_t>(cplx_[D - 1]);
+    backend_.memcpy2d(real_output, n * sizeof(FloatType), padded_input, pd * sizeof(FloatType),
+                      n * sizeof(FloatType), rprod(0, D - 1));
+  }
+
+  /// parafaft_r2c.hpp:333-339: unpadded real input (left untouched) to the
+  /// spectrum in `complex_output` (get_required_output_size() scalars, also
+  /// the working space).  Fast mode fuses the padding copy into the R2C pass.
+  void forward(const FloatType *real_input, Complex *complex_output) {
+    if (mode_ == Mode::Fast &&
+        fused_oop(PFX_FORWARD, const_cast<FloatType *>(real_input), complex_output, 1.0))
+      return;
+    copy_real_to_padded(real_input, reinterpret_cast<FloatType *>(complex_output));
+    forward_in_place(reinterpret_cast<FloatType *>(complex_output));
+  }
+
+  /// parafaft_r2c.hpp:446-453: the spectrum in `complex_input` (working
+  /// space, destroyed) to the real output.  Fast mode fuses the padding copy
+  /// into the C2R pass.
+  void backward(Complex *complex_input, FloatType *real_output, double scale = 1.0) {
+    if (mode_ == Mode::Fast && fused_oop(PFX_BACKWARD, real_output, complex_input, scale)) return;
+    backward_in_place(reinterpret_cast<FloatType *>(complex_input), scale);
+    copy_padded_to_real(reinterpret_cast<FloatType *>(complex_input), real_output);
+    backend_.sync();
+  }
+
   Backend &backend() { return backend_; }
 
 private:
+  // false when a real-axis plan has no tiled kernel (then copy + in place)
+  bool fused_oop(int direction, FloatType *real, Complex *padded, double scale) {
+    const int rc =
+        pfx_ndplan_execute_r2c_oop(nd_, direction, real, padded, scale, backend_.stream());
+    if (rc == PFX_ERR_UNSUPPORTED) return false;
+    parafaft::check_pfx(rc, "pfx_ndplan_execute_r2c_oop");
+    backend_.sync();
+    return true;
+  }
   void plan_fast(FFTPlanFlag flag, std::size_t bytes) { plan_nd(nd_, flag, bytes, backend_.stream()); }
   std::size_t cprod(int from, int to) const {
     std::size_t p = 1;

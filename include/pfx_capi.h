@@ -112,6 +112,21 @@ int pfx_execute_scaled(pfx_plan plan, int direction, void *data, double scale,
                        pfx_stream stream);
 
 /*
+ * Out-of-place forms of R2C / C2R plans: the real rows are unpadded,
+ * `real_dist` scalars apart (even, >= the length); the complex side is the
+ * plan's padded layout.  The R2C's first pass reads the real rows and the
+ * C2R's last pass writes them, so the padding copy of ParaFaFT_R2C::forward
+ * (copy_real_to_padded, parafaft_r2c.hpp:333-339, 804-823) and ::backward
+ * (copy_padded_to_real, :446-453, 825-847) is fused into the transform.
+ * The C2R form leaves `padded_in` unchanged.  Plans without a tiled kernel
+ * return PFX_ERR_UNSUPPORTED (copy, then transform in place).
+ */
+int pfx_execute_r2c_oop(pfx_plan plan, const void *real_in, int64_t real_dist,
+                        void *padded_out, double scale, pfx_stream stream);
+int pfx_execute_c2r_oop(pfx_plan plan, void *padded_in, void *real_out, int64_t real_dist,
+                        double scale, pfx_stream stream);
+
+/*
  * One axis of a row-major block split into per-peer segments, in the order
  * the pencil exchange packs it (parafaft_common.hpp:576-590): element
  * (lead, e, trail), e on the split axis, is complex element
@@ -255,6 +270,17 @@ int pfx_ndplan_create(pfx_ndplan *plan, int precision, int kind, int ndim, const
 /* scale: multiplies the result inside the last pass (1.0 = unnormalised). */
 int pfx_ndplan_execute(pfx_ndplan plan, int direction, void *data, double scale,
                        pfx_stream stream);
+
+/*
+ * R2C N-D plans, out of place (ParaFaFT_R2C::forward(real, complex) and
+ * ::backward(complex, real), parafaft_r2c.hpp:333-339, 446-453): forward reads
+ * the unpadded real array `real` and leaves the spectrum in `padded`;
+ * backward transforms `padded` in place (destroying it) and writes the real
+ * result to `real`.  The padding copies are fused into the first R2C and the
+ * last C2R pass.
+ */
+int pfx_ndplan_execute_r2c_oop(pfx_ndplan plan, int direction, void *real, void *padded,
+                               double scale, pfx_stream stream);
 int pfx_ndplan_destroy(pfx_ndplan plan);
 
 /* Tunes every stage plan (pfx_plan_tune) on `scratch`, a device buffer of the

@@ -56,6 +56,8 @@ SIGNATURES = [
     ("pfx_execute", _i, [_vp, _i, _vp, _vp]),
     ("pfx_execute_scaled", _i, [_vp, _i, _vp, _d, _vp]),
     ("pfx_execute_segmented", _i, [_vp, _i, _vp, _vp, _vp, _d, _vp]),
+    ("pfx_execute_r2c_oop", _i, [_vp, _vp, _i64, _vp, _d, _vp]),
+    ("pfx_execute_c2r_oop", _i, [_vp, _vp, _vp, _i64, _d, _vp]),
     ("pfx_plan_destroy", _i, [_vp]),
     ("pfx_plan_describe", _i, [_vp, C.c_char_p, _sz, _ip, _ip]),
     ("pfx_plan_tune", _i, [_vp, _i, _vp, _vp]),
@@ -77,6 +79,7 @@ SIGNATURES = [
     ("pfx_scale", _i, [_vp, _i64, _i, _d, _vp]),
     ("pfx_ndplan_create", _i, [_vpp, _i, _i, _i, _ip]),
     ("pfx_ndplan_execute", _i, [_vp, _i, _vp, _d, _vp]),
+    ("pfx_ndplan_execute_r2c_oop", _i, [_vp, _i, _vp, _vp, _d, _vp]),
     ("pfx_ndplan_destroy", _i, [_vp]),
     ("pfx_ndplan_stage", _i, [_vp, _i, _vpp]),
     ("pfx_ndplan_tune", _i, [_vp, _i, _vp, _vp]),

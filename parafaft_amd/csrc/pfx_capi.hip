@@ -363,7 +363,33 @@ int pfx_plan_create_c2r(pfx_plan *plan, int precision, int64_t length, int64_t b
   return plan_create(plan, precision, PFX_C2R, length, batch, stride, dist_scalars, 1, 0);
 }
 
+// Launch a plan's kernel; `real_oop` / `real_dist` (scalars) make an R2C
+// plan load its rows from an unpadded real array, or a C2R plan store them
+// into one, instead of the padded rows of `data`.
+static int execute_impl(pfx_plan p, int direction, void *data, double scale, pfx_stream stream,
+                        void *real_oop, int64_t real_dist);
+
 int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_stream stream) {
+  return execute_impl(p, direction, data, scale, stream, nullptr, 0);
+}
+
+int pfx_execute_r2c_oop(pfx_plan p, const void *real_in, int64_t real_dist, void *padded_out,
+                        double scale, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (p->kind != PFX_R2C) return fail(PFX_ERR_INVALID_ARG, "not an R2C plan");
+  return execute_impl(p, PFX_FORWARD, padded_out, scale, stream, const_cast<void *>(real_in),
+                      real_dist);
+}
+
+int pfx_execute_c2r_oop(pfx_plan p, void *padded_in, void *real_out, int64_t real_dist,
+                        double scale, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (p->kind != PFX_C2R) return fail(PFX_ERR_INVALID_ARG, "not a C2R plan");
+  return execute_impl(p, PFX_BACKWARD, padded_in, scale, stream, real_out, real_dist);
+}
+
+static int execute_impl(pfx_plan p, int direction, void *data, double scale, pfx_stream stream,
+                        void *real_oop, int64_t real_dist) {
   if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
   if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
     return fail(PFX_ERR_INVALID_ARG, "direction must be PFX_FORWARD or PFX_BACKWARD");
@@ -374,11 +400,26 @@ int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_
   const int64_t total = p->inner * p->outer;
   if (total == 0) return PFX_SUCCESS;
   if (!data) return fail(PFX_ERR_INVALID_ARG, "data is null");
+  if (real_oop) {
+    if (!p->kernel)
+      return fail(PFX_ERR_UNSUPPORTED,
+                  "out-of-place R2C/C2R needs a tiled kernel (even length with a radix plan)");
+    if (real_dist < p->length || real_dist % 2 != 0 || (p->outer > 1))
+      return fail(PFX_ERR_INVALID_ARG,
+                  "real rows need an even distance of at least the length (got %lld)",
+                  (long long)real_dist);
+  }
   hipStream_t s = static_cast<hipStream_t>(stream);
   const bool real = p->kind != PFX_C2C;
   if (p->kernel) {
     StockhamArgs a;
-    a.src = nullptr;
+    if (real_oop && p->kind == PFX_R2C) {
+      a.src = real_oop;               // complex pairs of the unpadded real rows
+      a.src_dist = real_dist / 2;
+    } else if (real_oop) {
+      a.dst = real_oop;
+      a.dst_dist = real_dist / 2;
+    }
     a.data = data;
     a.tw = p->tw;
     a.ptw = p->ptw;
@@ -489,6 +530,8 @@ int pfx_execute_segmented(pfx_plan p, int direction, void *data, const pfx_segma
               static_cast<const char *>(in->segments[0]) + (size_t)in->start[q] * in->trailing * esz;
     if (plain) {
       a.src = in->segments[0];
+      a.src_dist = a.dist;
+      a.src_dist_outer = a.dist_outer;
       a.in_seg = 0;
     }
   }
@@ -940,25 +983,40 @@ int pfx_ndplan_set_chunk(pfx_ndplan p, int planes) {
   return make_chunk_plans(p, cs, planes);
 }
 
-int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pfx_stream stream) {
+// One N-D transform.  `real` (R2C plans only): the unpadded real array, read
+// by the first R2C pass (forward) or written by the last C2R pass (backward)
+// in place of the padded rows of `data`.
+static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_stream stream,
+                  void *real) {
   if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
   if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
     return fail(PFX_ERR_INVALID_ARG, "bad direction");
   const int D = p->ndim;
+  const int64_t nlast = p->shape[D - 1];
+  // the R2C (forward) or C2R (backward) stage of the real axis
+  auto exec = [&](pfx_plan pl, char *base, double sc, char *real_base) -> int {
+    if (real_base && pl->kind != PFX_C2C)
+      return execute_impl(pl, direction, base, sc, stream, real_base, nlast);
+    return pfx_execute_scaled(pl, direction, base, sc, stream);
+  };
   if (p->chunk > 0) {
     const int64_t esz = base_prec(p->precision) ? 16 : 8;
     const int nchunks = p->shape[0] / p->chunk;
     const size_t chunk_bytes = (size_t)(p->chunk * p->plane_elems * esz);
+    int64_t real_plane = 1;
+    for (int i = 1; i < D; ++i) real_plane *= p->shape[i];
+    const size_t chunk_real_bytes = (size_t)(p->chunk * real_plane * (esz / 2));
     auto run_chunks = [&](double last_scale) -> int {
       for (int c = 0; c < nchunks; ++c) {
         char *base = static_cast<char *>(data) + (size_t)c * chunk_bytes;
+        char *rbase = real ? static_cast<char *>(real) + (size_t)c * chunk_real_bytes : nullptr;
         for (int k = 0; k < D - 1; ++k) {
           // forward: stages 0..D-2; backward: D-2..0 (C2R last for R2C)
           const int st = direction == PFX_FORWARD ? k : D - 2 - k;
           pfx_plan pl = p->cstages[st];
           if (p->kind == PFX_R2C && st == 0 && direction == PFX_BACKWARD) pl = p->cstages[D - 1];
           const double sc = (k == D - 2) ? last_scale : 1.0;
-          int rc = pfx_execute_scaled(pl, direction, base, sc, stream);
+          int rc = exec(pl, base, sc, rbase);
           if (rc != PFX_SUCCESS) return rc;
         }
       }
@@ -978,10 +1036,33 @@ int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pf
     pfx_plan pl = p->stages[s];
     if (p->kind == PFX_R2C && s == 0 && direction == PFX_BACKWARD) pl = p->stages[D];
     const double sc = (k == D - 1) ? scale : 1.0;
-    int rc = pfx_execute_scaled(pl, direction, data, sc, stream);
+    int rc = exec(pl, static_cast<char *>(data), sc, static_cast<char *>(real));
     if (rc != PFX_SUCCESS) return rc;
   }
   return PFX_SUCCESS;
+}
+
+int pfx_ndplan_execute(pfx_ndplan p, int direction, void *data, double scale, pfx_stream stream) {
+  return nd_run(p, direction, data, scale, stream, nullptr);
+}
+
+int pfx_ndplan_execute_r2c_oop(pfx_ndplan p, int direction, void *real, void *padded,
+                               double scale, pfx_stream stream) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (p->kind != PFX_R2C) return fail(PFX_ERR_INVALID_ARG, "not an R2C plan");
+  if (!real || !padded) return fail(PFX_ERR_INVALID_ARG, "null buffer");
+  if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "bad direction");
+  // Decided before any stage runs: a backward that found out at its last
+  // (C2R) stage would leave the earlier stages applied to `padded`.
+  const int D = p->ndim;
+  const pfx_plan real_stage = direction == PFX_FORWARD
+                                  ? (p->chunk > 0 ? p->cstages[0] : p->stages[0])
+                                  : (p->chunk > 0 ? p->cstages[D - 1] : p->stages[D]);
+  if (!real_stage->kernel)
+    return fail(PFX_ERR_UNSUPPORTED,
+                "out-of-place R2C/C2R needs a tiled kernel (even length with a radix plan)");
+  return nd_run(p, direction, padded, scale, stream, real);
 }
 
 int pfx_ndplan_destroy(pfx_ndplan p) {

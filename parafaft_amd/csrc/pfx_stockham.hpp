@@ -86,8 +86,13 @@ __device__ __forceinline__ V2<T>* seg_addr(KSegMap* m, uint32_t lead, int64_t tr
 
 struct StockhamArgs {
   void* data;            // complex view of the buffer (R2C/C2R: padded rows)
-  const void* src;       // out of place: first-pass loads come from here, at
-                         // the same offsets (nullptr: in place, from data)
+  const void* src = nullptr;  // out of place: first-pass loads come from here
+                              // (nullptr: in place, from data), transform b at
+                              // (b / inner) src_dist_outer + (b % inner) src_dist
+  int64_t src_dist = 0, src_dist_outer = 0;
+  void* dst = nullptr;        // out of place: last-pass stores go here (nullptr:
+                              // into data), at dst_dist / dst_dist_outer likewise
+  int64_t dst_dist = 0, dst_dist_outer = 0;
   const void* tw;        // W_L^i, i in [0, L): L = N (C2C) or 2N (R2C/C2R)
   const void* ptw;       // per-pass table of the LDS FFT: entry m*Ns + kk - 1
                          // holds W_{Ns R}^{kk m} for the pass with span Ns
@@ -336,19 +341,31 @@ __device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a
   const int64_t i = b - o * a.inner;
   return o * a.dist_outer + i * a.dist;
 }
+__device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a, int64_t dist,
+                                                int64_t dist_outer) {
+  const int64_t o = b / a.inner;
+  const int64_t i = b - o * a.inner;
+  return o * dist_outer + i * dist;
+}
 
 template <typename T, bool SEG, bool TAB>
 __device__ __forceinline__ void init_global(GlobalPtr<T, SEG, TAB>& g, const StockhamArgs& a, int64_t b) {
   g.valid = b < a.total;
-  const V2<T>* in_base = static_cast<const V2<T>*>(a.src ? a.src : a.data);
   if constexpr (!SEG) {
     const int64_t off = g.valid ? xform_offset(b, a) : 0;
-    g.p = static_cast<V2<T>*>(a.data) + off;
-    g.pin = in_base + off;
+    g.p = a.dst ? static_cast<V2<T>*>(a.dst) +
+                      (g.valid ? xform_offset(b, a, a.dst_dist, a.dst_dist_outer) : 0)
+                : static_cast<V2<T>*>(a.data) + off;
+    g.pin = a.src ? static_cast<const V2<T>*>(a.src) +
+                        (g.valid ? xform_offset(b, a, a.src_dist, a.src_dist_outer) : 0)
+                  : static_cast<const V2<T>*>(a.data) + off;
   } else {
     const int64_t off = g.valid && !(a.in_seg && a.out_seg) ? xform_offset(b, a) : 0;
+    const bool v = g.valid && !a.in_seg;
     g.p = static_cast<V2<T>*>(a.data) + off;
-    g.pin = in_base + off;
+    g.pin = a.src ? static_cast<const V2<T>*>(a.src) +
+                        (v ? xform_offset(b, a, a.src_dist, a.src_dist_outer) : 0)
+                  : static_cast<const V2<T>*>(a.data) + off;
     // Only single-plan kernels run segmented, and their one argument (the
     // StockhamArgs) sits at offset 0 of the kernel-argument segment; taking
     // the address of the parameter itself would copy it to scratch.

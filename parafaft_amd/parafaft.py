@@ -43,6 +43,16 @@ class _NdPlan:
         A.check(A.load().pfx_ndplan_execute(self.handle, direction, ptr(data), scale, stream),
                 "pfx_ndplan_execute")
 
+    def execute_r2c_oop(self, direction, real, padded, scale, stream) -> bool:
+        """Out-of-place R2C/C2R with the padding copy fused into the real-axis
+        pass; False when the plan has no tiled kernel for it."""
+        rc = A.load().pfx_ndplan_execute_r2c_oop(self.handle, direction, ptr(real), ptr(padded),
+                                                  scale, stream)
+        if rc == A.PFX_ERR_UNSUPPORTED:
+            return False
+        A.check(rc, "pfx_ndplan_execute_r2c_oop")
+        return True
+
     def plan(self, flag: FFTPlanFlag, nbytes: int, stream):
         """Measure / Patient planning (kernels and chunked or stage-by-stage
         schedule) on a temporary device buffer of the array's size, freed
@@ -275,6 +285,48 @@ class ParaFaFT_R2C:
             src, dst = dst, src
         if sync:
             self.backend.sync()
+
+    def copy_real_to_padded(self, real_input, padded_output):
+        """parafaft_r2c.hpp:804-823: contiguous real rows into padded rows of
+        2(N/2+1) scalars, one 2D copy."""
+        n = self.global_real_shape[-1]
+        rsz = self.esz // 2
+        rows = _prod(self.stage_shapes[0][:self.D - 1])
+        self.backend.memcpy2d(padded_output, 2 * self.global_complex_shape[-1] * rsz,
+                              real_input, n * rsz, n * rsz, rows)
+
+    def copy_padded_to_real(self, padded_input, real_output):
+        """parafaft_r2c.hpp:825-847: padded rows back to contiguous real rows."""
+        n = self.global_real_shape[-1]
+        rsz = self.esz // 2
+        rows = _prod(self.stage_shapes[0][:self.D - 1])
+        self.backend.memcpy2d(real_output, n * rsz, padded_input,
+                              2 * self.global_complex_shape[-1] * rsz, n * rsz, rows)
+
+    def forward(self, real_input, complex_output, sync: bool = True):
+        """parafaft_r2c.hpp:333-339: real input (unpadded, untouched) to the
+        spectrum in `complex_output` (get_required_output_size() scalars, also
+        the working space).  Fast mode fuses the padding copy into the R2C
+        pass."""
+        if self.fast and self._nd.execute_r2c_oop(A.PFX_FORWARD, real_input, complex_output,
+                                                  1.0, self.backend.stream):
+            if sync:
+                self.backend.sync()
+            return
+        self.copy_real_to_padded(real_input, complex_output)
+        self.forward_in_place(complex_output, sync)
+
+    def backward(self, complex_input, real_output, scale: float = 1.0):
+        """parafaft_r2c.hpp:446-453: the spectrum in `complex_input` (used as
+        working space, destroyed) to the real output.  Fast mode fuses the
+        padding copy into the C2R pass."""
+        if self.fast and self._nd.execute_r2c_oop(A.PFX_BACKWARD, real_output, complex_input,
+                                                  scale, self.backend.stream):
+            self.backend.sync()
+            return
+        self.backward_in_place(complex_input, scale)
+        self.copy_padded_to_real(complex_input, real_output)
+        self.backend.sync()
 
     def backward_in_place(self, padded, scale: float = 1.0):
         """parafaft_r2c.hpp:473-542 (unnormalised unless scale != 1)."""
