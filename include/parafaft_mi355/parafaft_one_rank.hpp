@@ -20,6 +20,7 @@
 #define PARAFAFT_MI355_ONE_RANK_HPP
 
 #include <cstddef>
+#include <cstdio>
 #include <utility>
 
 #include "fft_backend_mi355.hpp"
@@ -89,6 +90,22 @@ public:
   void get_final_start(int start[D]) const { zero(start); }
   void get_domain_decomposition(int dec[D]) const {
     for (int i = 0; i < D; ++i) dec[i] = 1;
+  }
+  /// parafaft_c2c.hpp:439-445: the backend's own buffer when it handles the
+  /// distributed transform itself; this backend does not (nullptr).
+  Complex *get_buffer() { return nullptr; }
+  /// parafaft_c2c.hpp:463, 473-477: a (D-1)-dimensional grid, one rank at 0.
+  static constexpr int get_grid_ndims() { return D - 1; }
+  void get_grid_coords(int coords[D - 1]) const {
+    for (int i = 0; i < D - 1; ++i) coords[i] = 0;
+  }
+  /// parafaft_c2c.hpp:499-530: every stage's local shape (one rank: global).
+  void print_stage_shapes() const {
+    for (int stage = 0; stage < D; ++stage) {
+      std::printf("Stage %d: [", stage);
+      for (int i = 0; i < D; ++i) std::printf(i ? ", %d" : "%d", global_shape_[i]);
+      std::printf("]\n");
+    }
   }
 
   /// parafaft_c2c.hpp:539-602 (unnormalised).
@@ -231,6 +248,34 @@ public:
   }
   void get_complex_global_start(int s[D]) const {
     for (int i = 0; i < D; ++i) s[i] = 0;
+  }
+  /// parafaft_r2c.hpp:712-717, 726-742: one rank, a (D-1)-dimensional grid.
+  void get_domain_decomposition(int dec[D]) const {
+    for (int i = 0; i < D; ++i) dec[i] = 1;
+  }
+  static constexpr int get_grid_ndims() { return D - 1; }
+  void get_grid_coords(int coords[D - 1]) const {
+    for (int i = 0; i < D - 1; ++i) coords[i] = 0;
+  }
+  /// parafaft_r2c.hpp:773-790: backend-owned buffers (none here: nullptr).
+  FloatType *get_real_buffer() { return nullptr; }
+  Complex *get_buffer() { return nullptr; }
+  /// parafaft_r2c.hpp:664-710: global real and complex shapes, and every
+  /// stage's input and output shape (one rank: the global ones).
+  void print_shapes() const {
+    auto line = [](const char *what, const int *v) {
+      std::printf("%s: [", what);
+      for (int i = 0; i < D; ++i) std::printf(i ? ", %d" : "%d", v[i]);
+      std::printf("]\n");
+    };
+    line("Global real shape", real_);
+    line("Global complex shape", cplx_);
+    for (int stage = 0; stage < D; ++stage) {
+      std::printf("Stage %d ", stage);
+      line(stage == 0 ? "input shape (real)" : "input shape", stage == 0 ? real_ : cplx_);
+      std::printf("Stage %d ", stage);
+      line("output shape (complex)", cplx_);
+    }
   }
 
   /// parafaft_r2c.hpp:360-430.

@@ -133,6 +133,22 @@ class ParaFaFT_C2C:
     def get_final_start(self):
         return [0] * self.D
 
+    def get_domain_decomposition(self):
+        """parafaft_c2c.hpp:453-458 (one rank)."""
+        return [1] * self.D
+
+    def get_grid_ndims(self):
+        """parafaft_c2c.hpp:463: the grid has D - 1 dimensions."""
+        return self.D - 1
+
+    def get_grid_coords(self):
+        """parafaft_c2c.hpp:473-477 (one rank: the origin)."""
+        return [0] * (self.D - 1)
+
+    def get_buffer(self):
+        """parafaft_c2c.hpp:439-445: no backend-owned buffer here."""
+        return None
+
     def _perform_fft_on_buffer(self, stage, axis, direction, buf, scale=1.0):
         D = self.D
         if axis == D - 1 or axis == 0:
@@ -250,6 +266,23 @@ class ParaFaFT_R2C:
 
     def get_complex_global_start(self):
         return [0] * self.D
+
+    def get_domain_decomposition(self):
+        """parafaft_r2c.hpp:712-717 (one rank)."""
+        return [1] * self.D
+
+    def get_grid_ndims(self):
+        return self.D - 1
+
+    def get_grid_coords(self):
+        return [0] * (self.D - 1)
+
+    def get_real_buffer(self):
+        """parafaft_r2c.hpp:773-790: no backend-owned buffers here."""
+        return None
+
+    def get_buffer(self):
+        return None
 
     def _perform_fft_on_buffer(self, stage, axis, direction, buf, scale=1.0):
         if axis == 0:

@@ -99,6 +99,19 @@ static void r2c_case(int N, int shape_id, Mode mode,
 
   using B = parafaft::MI355Backend<T, SM>;
   parafaft_mi355::ParaFaFT_R2C<D, B> fft(shape, flag, mode);
+  {  // one rank of a (D-1)-dimensional grid (parafaft_r2c.hpp:712-790)
+    int dec[D], coords[D > 1 ? D - 1 : 1];
+    fft.get_domain_decomposition(dec);
+    fft.get_grid_coords(coords);
+    bool acc_ok = decltype(fft)::get_grid_ndims() == D - 1 && fft.get_buffer() == nullptr &&
+                  fft.get_real_buffer() == nullptr;
+    for (int i = 0; i < D; ++i) acc_ok = acc_ok && dec[i] == 1;
+    for (int i = 0; i < D - 1; ++i) acc_ok = acc_ok && coords[i] == 0;
+    if (!acc_ok) {
+      std::printf("    R2C accessors wrong\n");
+      ++failures;
+    }
+  }
   typename B::Buffer d(fft.get_required_output_size());
   sync_copy(d.data(), pad.data(), pad.size() * sizeof(T));
   fft.forward_in_place(d.data());

@@ -33,6 +33,10 @@ def test_c2c_nd_matrix(prec, fast, D, N, shape_id):
     fft = P.ParaFaFT_C2C([N] * D, precision=prec, fast=fast)
     assert fft.get_local_shape() == [N] * D
     assert fft.get_required_output_size() == N ** D
+    # one rank of a (D-1)-dimensional grid (parafaft_c2c.hpp:439-477)
+    assert fft.get_domain_decomposition() == [1] * D
+    assert fft.get_grid_ndims() == D - 1 and fft.get_grid_coords() == [0] * (D - 1)
+    assert fft.get_buffer() is None
     d = to_dev(x.astype(cdt))
     fft.forward(d)
     y = to_host(d)
