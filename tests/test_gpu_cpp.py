@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "cpp", "_build")
 
 
-@pytest.mark.parametrize("exe", ["test_mi355_backend", "test_mi355_nd"])
+@pytest.mark.parametrize("exe", ["test_mi355_backend", "test_mi355_nd", "test_pfxvector"])
 def test_cpp_executable(exe):
     path = os.path.join(BUILD, exe)
     if not os.path.exists(path):
