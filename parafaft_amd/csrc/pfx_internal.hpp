@@ -49,6 +49,11 @@ void launch_c2c_wide(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
   c2c_wide_kernel<T, TPT, C, ROWL, false, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
 }
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
+void launch_c2c_wide_nt(const StockhamArgs &a, hipStream_t s) {
+  const int64_t nb = (a.total + C - 1) / C;
+  c2c_wide_nt_kernel<T, TPT, C, ROWL, Rs...><<<dim3((unsigned)nb), dim3(TPT * C), 0, s>>>(a);
+}
 template <typename T, int TPT, int C, int... Rs>
 void launch_r2c(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + C - 1) / C;
@@ -75,6 +80,8 @@ const KernelEntry *table_f64_col(int *count);
 const KernelEntry *table_f64_real(int *count);
 const KernelEntry *table_split(int *count);
 const KernelEntry *table_f32_w3232(int *count);
+const KernelEntry *table_nt(int *count);
+const KernelEntry *table_f32_w3232nt(int *count);
 const GroupLaunchFn *group_f32_w3232(int *count);
 const LaunchFn *seg_f32_w3232(int *count);
 const GroupLaunchFn *group_f32_row(int *count);

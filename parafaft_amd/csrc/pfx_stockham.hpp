@@ -237,7 +237,10 @@ template <int N> struct SegTabs {
   }
 };
 
-template <typename T, bool SEG = false, bool TAB = false> struct GlobalPtr {
+// NT: non-temporal HBM accesses (streaming, no cache allocation) for a
+// pass whose data is not read again soon; measured 4% faster on the
+// 1024^3 axis-0 column pattern when both loads and stores stream.
+template <typename T, bool SEG = false, bool TAB = false, bool NT = false> struct GlobalPtr {
   V2<T>* p;
   const V2<T>* pin;          // load side (== p in place)
   int64_t stride;
@@ -245,11 +248,28 @@ template <typename T, bool SEG = false, bool TAB = false> struct GlobalPtr {
   bool valid;
   T scale;
   bool use_scale;
-  __device__ __forceinline__ V2<T> load(int e) const { return pin[(int64_t)e * stride]; }
-  __device__ __forceinline__ void store(int e, V2<T> v) const { p[(int64_t)e * stride] = v; }
+  typedef T E2 __attribute__((ext_vector_type(2)));
+  __device__ __forceinline__ V2<T> load(int e) const {
+    if constexpr (NT) {
+      const E2 x = __builtin_nontemporal_load(reinterpret_cast<const E2*>(pin + (int64_t)e * stride));
+      return mk<T>(x.x, x.y);
+    } else {
+      return pin[(int64_t)e * stride];
+    }
+  }
+  __device__ __forceinline__ void store(int e, V2<T> v) const {
+    if constexpr (NT) {
+      E2 x;
+      x.x = v.x;
+      x.y = v.y;
+      __builtin_nontemporal_store(x, reinterpret_cast<E2*>(p + (int64_t)e * stride));
+    } else {
+      p[(int64_t)e * stride] = v;
+    }
+  }
 };
 
-template <typename T, bool TAB> struct GlobalPtr<T, true, TAB> {
+template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
   V2<T>* p;
   const V2<T>* pin;
   int64_t stride;
@@ -315,7 +335,7 @@ template <typename T, bool TAB> struct GlobalPtr<T, true, TAB> {
 // Build the LDS tables of the maps a segmented launch uses and point g at
 // them.  Every thread of the workgroup calls this (it ends in a barrier).
 template <typename T, int N>
-__device__ __forceinline__ void seg_tables(GlobalPtr<T, true, true>& g, unsigned char* mem) {
+__device__ __forceinline__ void seg_tables(GlobalPtr<T, true, true, false>& g, unsigned char* mem) {
   using ST = SegTabs<N>;
   if (g.in_map) ST::fill(mem, 0, g.in_map);
   if (g.out_map) ST::fill(mem, 1, g.out_map);
@@ -348,8 +368,9 @@ __device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a
   return o * dist_outer + i * dist;
 }
 
-template <typename T, bool SEG, bool TAB>
-__device__ __forceinline__ void init_global(GlobalPtr<T, SEG, TAB>& g, const StockhamArgs& a, int64_t b) {
+template <typename T, bool SEG, bool TAB, bool NT>
+__device__ __forceinline__ void init_global(GlobalPtr<T, SEG, TAB, NT>& g, const StockhamArgs& a,
+                                            int64_t b) {
   g.valid = b < a.total;
   if constexpr (!SEG) {
     const int64_t off = g.valid ? xform_offset(b, a) : 0;
@@ -767,8 +788,8 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
   }
 };
 
-template <typename T, int TPT, int C, bool ROWL, bool SEG, int... Rs>
-__device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+template <typename T, int TPT, int C, bool ROWL, bool SEG, bool NT, int... Rs>
+__device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
   constexpr int N = (Rs * ... * 1);
   constexpr int E = N / TPT;
   constexpr int R0 = first_of<Rs...>();
@@ -792,7 +813,7 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
     tile = tile_order(tile, ntiles, a.xcd_remap);
   }
   const int64_t b = tile * C + c;
-  GlobalPtr<T, SEG, TAB> g;
+  GlobalPtr<T, SEG, TAB, NT && !SEG> g;
   init_global(g, a, b);
   if constexpr (SEG) {
     g.lane_j = j;
@@ -818,8 +839,19 @@ __device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t til
 }
 
 template <typename T, int TPT, int C, bool ROWL, bool SEG, int... Rs>
+__device__ __forceinline__ void c2c_wide_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+  c2c_wide_body_x<T, TPT, C, ROWL, SEG, false, Rs...>(a, tile, ntiles);
+}
+
+template <typename T, int TPT, int C, bool ROWL, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_wide_kernel(StockhamArgs a) {
   c2c_wide_body<T, TPT, C, ROWL, SEG, Rs...>(a, blockIdx.x, gridDim.x);
+}
+
+// The same kernel with non-temporal HBM loads and stores (plain form only).
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2c_wide_nt_kernel(StockhamArgs a) {
+  c2c_wide_body_x<T, TPT, C, ROWL, false, true, Rs...>(a, blockIdx.x, gridDim.x);
 }
 
 // ---------------------------------------------------------------------------
