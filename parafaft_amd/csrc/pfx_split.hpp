@@ -369,6 +369,27 @@ __device__ __forceinline__ V2<float> split_pass2_twiddle(const V2<float>* tw, in
   return m > 0 ? t : mk<float>(1.f, 0.f);
 }
 
+// One complex<float> from / to HBM, streaming (non-temporal) when NT.
+typedef float f32x2_t __attribute__((ext_vector_type(2)));
+template <bool NT> __device__ __forceinline__ V2<float> split_ld(const V2<float>* p) {
+  if constexpr (NT) {
+    const f32x2_t x = __builtin_nontemporal_load(reinterpret_cast<const f32x2_t*>(p));
+    return mk<float>(x.x, x.y);
+  } else {
+    return *p;
+  }
+}
+template <bool NT> __device__ __forceinline__ void split_st(V2<float>* p, V2<float> v) {
+  if constexpr (NT) {
+    f32x2_t x;
+    x.x = v.x;
+    x.y = v.y;
+    __builtin_nontemporal_store(x, reinterpret_cast<f32x2_t*>(p));
+  } else {
+    *p = v;
+  }
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -478,7 +499,7 @@ template <int N, int C> struct SplitColLds {
   }
 };
 
-template <int MODE, int R3, int W, int CG, bool SW>
+template <int MODE, int R3, int W, int CG, bool SW, bool NT>
 __device__ __forceinline__ void split_col_body(const StockhamArgs& a, V2<float>* lds) {
   constexpr int N = 256 * R3, NB = N / 16, C = 16 * CG;
   constexpr int P1 = NB / W;  // pass-1 butterflies per lane
@@ -507,7 +528,7 @@ __device__ __forceinline__ void split_col_body(const StockhamArgs& a, V2<float>*
     for (int i = 0; i < P1; ++i)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const V2<float> x = ld[i * sW + r * sNB];
+        const V2<float> x = split_ld<NT>(ld + (i * sW + r * sNB));
         v[i][r] = SW ? swp<float>(x) : x;
       }
   }
@@ -557,18 +578,20 @@ __device__ __forceinline__ void split_col_body(const StockhamArgs& a, V2<float>*
 #pragma unroll
         for (int k2 = 0; k2 < R3; ++k2) {
           const V2<float> x = scl<float>(t[k2], sc);
-          sp[k * sW + r * s16 + k2 * s256] = SW ? swp<float>(x) : x;
+          split_st<NT>(sp + (k * sW + r * s16 + k2 * s256), SW ? swp<float>(x) : x);
         }
       }
     }
   }
 }
 
-template <int MODE, int R3, int W, int CG>
+// NT: streaming HBM loads and stores, for an axis-0 pass (data not read
+// again soon); a planner candidate beside the ordinary form.
+template <int MODE, int R3, int W, int CG, bool NT>
 __global__ void __launch_bounds__(64 * W * CG) c2c_split_col_kernel(StockhamArgs a) {
   __shared__ __align__(16) V2<float> lds[SplitColLds<256 * R3, 16 * CG>::size];
-  if (a.backward) split_col_body<MODE, R3, W, CG, true>(a, lds);
-  else split_col_body<MODE, R3, W, CG, false>(a, lds);
+  if (a.backward) split_col_body<MODE, R3, W, CG, true, NT>(a, lds);
+  else split_col_body<MODE, R3, W, CG, false, NT>(a, lds);
 }
 
 // Batch-contiguous layout with the exchange in R3 rounds.  Input m of

@@ -17,10 +17,10 @@ void launch_split_row(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + WPB - 1) / WPB;
   c2c_split_row_kernel<MODE, R3, WPB><<<dim3((unsigned)nb), dim3(64 * WPB), 0, s>>>(a);
 }
-template <int MODE, int R3, int W, int CG>
+template <int MODE, int R3, int W, int CG, bool NT = false>
 void launch_split_col(const StockhamArgs &a, hipStream_t s) {
   const int64_t nb = (a.total + 16 * CG - 1) / (16 * CG);
-  c2c_split_col_kernel<MODE, R3, W, CG><<<dim3((unsigned)nb), dim3(64 * W * CG), 0, s>>>(a);
+  c2c_split_col_kernel<MODE, R3, W, CG, NT><<<dim3((unsigned)nb), dim3(64 * W * CG), 0, s>>>(a);
 }
 template <int MODE, int R3, int W, int CG>
 void launch_split_colr(const StockhamArgs &a, hipStream_t s) {
@@ -43,6 +43,11 @@ template <int R3> constexpr Radices split_radices() {
   {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, 256 * R3 * 16 * CG * 8, 2,                  \
    "split-reg:16, 16, " #R3 "/w" #W, &launch_split_col<MODE, R3, W, CG>, split_radices<R3>(), \
    nullptr},
+// ... with streaming (non-temporal) HBM accesses, for axis-0 passes
+#define PFX_SPLIT_COLNT(MODE, R3, W, CG)                                                    \
+  {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, 256 * R3 * 16 * CG * 8, 0,                  \
+   "split-reg-nt:16, 16, " #R3 "/w" #W, &launch_split_col<MODE, R3, W, CG, true>,           \
+   split_radices<R3>(), nullptr},
 // ... exchanging in R3 rounds through two 256 x 16 CG buffers (N = 2048)
 #define PFX_SPLIT_COLR(MODE, R3, W, CG)                                                     \
   {MODE, KIND_C2C, 256 * R3, 1, 4 * W, 16 * CG, (R3 > 1 ? 2 : 1) * 256 * 16 * CG * 8, 2,     \
@@ -52,7 +57,8 @@ template <int R3> constexpr Radices split_radices() {
   PFX_SPLIT_ROW(MODE, 1, 4) PFX_SPLIT_ROW(MODE, 2, 4) PFX_SPLIT_ROW(MODE, 4, 4)             \
   PFX_SPLIT_ROW(MODE, 4, 2) PFX_SPLIT_ROW(MODE, 8, 4)                                       \
   PFX_SPLIT_COL(MODE, 1, 4, 2) PFX_SPLIT_COL(MODE, 2, 8, 2) PFX_SPLIT_COL(MODE, 4, 16, 1)   \
-  PFX_SPLIT_COL(MODE, 4, 8, 1) PFX_SPLIT_COLR(MODE, 8, 16, 1)
+  PFX_SPLIT_COL(MODE, 4, 8, 1) PFX_SPLIT_COLR(MODE, 8, 16, 1)                               \
+  PFX_SPLIT_COLNT(MODE, 4, 16, 1) PFX_SPLIT_COLNT(MODE, 4, 8, 1)
 
 #define PFX_SPLIT_ENTRY(MODE, COL, TILE, TPT, ...)                                         \
   {MODE, KIND_C2C, prod<__VA_ARGS__>(), COL, TPT, TILE,                                     \
