@@ -81,6 +81,7 @@ const KernelEntry *table_f64_real(int *count);
 const KernelEntry *table_split(int *count);
 const KernelEntry *table_f32_w3232(int *count);
 const KernelEntry *table_nt(int *count);
+const GroupLaunchFn *group_nt(int *count);
 const KernelEntry *table_f32_w3232nt(int *count);
 const GroupLaunchFn *group_f32_w3232(int *count);
 const LaunchFn *seg_f32_w3232(int *count);

@@ -14,6 +14,10 @@ template <typename T, int TPT, int C, bool ROWL, int... Rs>
 void group_c2c_wide(const GroupArgs &g, int64_t blocks, hipStream_t s) {
   c2c_wide_group_kernel<T, TPT, C, ROWL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
 }
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
+void group_c2c_wide_nt(const GroupArgs &g, int64_t blocks, hipStream_t s) {
+  c2c_wide_nt_group_kernel<T, TPT, C, ROWL, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);
+}
 template <typename T, int TPT, int C, int... Rs>
 void group_r2c(const GroupArgs &g, int64_t blocks, hipStream_t s) {
   r2c_group_kernel<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(g);

@@ -889,6 +889,13 @@ __global__ void __launch_bounds__(TPT * C) c2c_wide_group_kernel(GroupArgs ga) {
   c2c_wide_body<T, TPT, C, ROWL, false, Rs...>(ga.jobs[jb], blockIdx.x - s0,
                                               ga.start[jb + 1] - s0);
 }
+template <typename T, int TPT, int C, bool ROWL, int... Rs>
+__global__ void __launch_bounds__(TPT * C) c2c_wide_nt_group_kernel(GroupArgs ga) {
+  const int jb = find_job(ga, blockIdx.x);
+  const int64_t s0 = ga.start[jb];
+  c2c_wide_body_x<T, TPT, C, ROWL, false, true, Rs...>(ga.jobs[jb], blockIdx.x - s0,
+                                                      ga.start[jb + 1] - s0);
+}
 template <typename T, int TPT, int C, int... Rs>
 __global__ void __launch_bounds__(TPT * C) r2c_group_kernel(GroupArgs ga) {
   const int jb = find_job(ga, blockIdx.x);

@@ -1,9 +1,8 @@
-// Non-temporal variants of the batch-contiguous register-resident kernels
-// that Measure planning picks for the axis-0 pass (the last pass of an N-D
-// forward, whose data is not read again soon): 1024^3 and 512^3, FP32 and
-// FP64.  Table only (no grouped or segmented form); lowest priority, so only
-// the planner's timing selects them.
+// Table entries of the streaming (non-temporal) batch-contiguous kernels of
+// pfx_nt_list.inc.  Their HBM loads and stores do not keep the data in L2;
+// they have the lowest priority, so only the planner's timing selects them.
 #include "pfx_internal.hpp"
+#include "pfx_nt_list.inc"
 
 namespace pfx {
 
@@ -12,11 +11,7 @@ namespace pfx {
    (int)(C * prod<__VA_ARGS__>() * sizeof(T)), 0, "wide-nt:" #__VA_ARGS__,                    \
    &launch_c2c_wide_nt<T, TPT, C, false, __VA_ARGS__>, radices<__VA_ARGS__>(), nullptr, nullptr},
 
-static const KernelEntry kTable[] = {
-    PFX_NT_ENTRY(float, 16, 32, 8, 8, 8)
-    PFX_NT_ENTRY(double, 64, 16, 8, 8, 8)
-    PFX_NT_ENTRY(double, 64, 16, 16, 16, 4)
-};
+static const KernelEntry kTable[] = {PFX_NT_LIST(PFX_NT_ENTRY)};
 
 const KernelEntry *table_nt(int *count) {
   *count = (int)(sizeof(kTable) / sizeof(kTable[0]));

@@ -220,3 +220,40 @@ def test_batch_chunked_schedule(monkeypatch):
         outs[mode] = to_host(d)
         _check(SMALL, outs[mode], offs, refs, prec, cdt)
     assert np.array_equal(outs["chunked"].view(np.uint8), outs["each"].view(np.uint8))
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_batch_streaming_grouped(monkeypatch, prec):
+    """The streaming (non-temporal) batch-contiguous kernels in their grouped
+    form: every column pass with a streaming variant is forced onto it
+    (PFX_KERNEL_MATCH), and the grouped schedule matches one launch per stage
+    bit for bit and the oracle."""
+    import parafaft_amd as P
+    from parafaft_amd import backend as B
+    from parafaft_amd.batch import Batch
+    items = ([("c2c", (512, 64)), ("c2c", (512, 8, 8)), ("c2c", (1024, 36)), ("c2c", (512, 40)),
+              ("c2c", (96, 512))] if prec == "f32" else
+             [("c2c", (1024, 20)), ("c2c", (512, 24)), ("c2c", (1024, 8, 4))])
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    rdt = np.float32 if prec == "f32" else np.float64
+    esz = np.dtype(cdt).itemsize
+    B.tune_cache_clear()   # a geometry tuned by an earlier test keeps its choice
+    monkeypatch.setenv("PFX_KERNEL_MATCH", "wide-nt:")
+    for kind, shape in items[:3]:
+        plan = P.Plan.c2c(prec, shape[0], int(np.prod(shape[1:])), int(np.prod(shape[1:])), 1)
+        assert "wide-nt" in plan.describe()[0], (shape, plan.describe())
+    offs, total = _layout(items, cdt)
+    xs = _inputs(items, 310)
+    host = _pack_inputs(items, xs, offs, total, cdt, rdt)
+    refs = _refs(items, [x.astype(cdt).astype(np.complex128) for x in xs])
+    d = to_dev(host)
+    b = Batch(prec, [(k, s, d.data_ptr() + o * esz) for (k, s), o in zip(items, offs)])
+    monkeypatch.delenv("PFX_KERNEL_MATCH")
+    outs = {}
+    for mode in ("grouped", "each"):
+        d.copy_(torch().from_numpy(host).to("cuda"))
+        torch().cuda.synchronize()
+        (b.execute if mode == "grouped" else b.execute_each)(0)
+        outs[mode] = to_host(d)
+        _check(items, outs[mode], offs, refs, prec, cdt)
+    assert np.array_equal(outs["grouped"].view(np.uint8), outs["each"].view(np.uint8))
