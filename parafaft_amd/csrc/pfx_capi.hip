@@ -220,6 +220,9 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
     return fail(PFX_ERR_UNSUPPORTED, "unknown precision %d", precision);
   if (length <= 0) return fail(PFX_ERR_INVALID_ARG, "length must be positive (got %lld)", (long long)length);
   if (inner < 0 || outer < 0) return fail(PFX_ERR_INVALID_ARG, "batch must be non-negative");
+  if (inner > 0 && outer > INT64_MAX / inner)
+    return fail(PFX_ERR_INVALID_ARG, "batch %lld x %lld overflows", (long long)inner,
+                (long long)outer);
   if (stride <= 0) return fail(PFX_ERR_INVALID_ARG, "stride must be positive (got %lld)", (long long)stride);
   if ((kind == PFX_R2C || kind == PFX_C2R) && stride != 1)
     return fail(PFX_ERR_UNSUPPORTED, "R2C/C2R plans support stride 1 only (got %lld)", (long long)stride);
@@ -227,9 +230,7 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
     return fail(PFX_ERR_INVALID_ARG, "padded dist %lld is below 2*(n/2+1)", (long long)dist);
   if ((kind == PFX_R2C || kind == PFX_C2R) && (dist % 2) != 0 && inner > 1)
     return fail(PFX_ERR_INVALID_ARG, "padded dist must be even (complex-aligned rows)");
-  const int64_t total = inner * outer;
-  if (total > (int64_t)0x7fffffff * 1)
-    ;  // the grid is sized per tile below; checked there
+  const int64_t total = inner * outer;  // the grid is sized per tile and checked below
 
   pfx_plan p = new (std::nothrow) pfx_plan_t;
   if (!p) return fail(PFX_ERR_ALLOC, "out of host memory");

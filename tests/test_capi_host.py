@@ -95,3 +95,15 @@ def test_no_cpu_fallback_in_product():
             if f.endswith(".py"):
                 src = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in src.replace("ORACLE", ""), f
+
+
+def test_batch_product_overflow_rejected():
+    """A two-level batch whose transform count overflows int64 is refused on
+    the host, before any allocation or device work."""
+    import parafaft_amd as P
+    L = P.load_library()
+    h = C.c_void_p()
+    rc = L.pfx_plan_create_c2c_many(C.byref(h), 0, 16, 1 << 40, 1, 16, 1 << 40, 1 << 44)
+    assert rc == 1
+    assert h.value is None
+    assert b"overflows" in L.pfx_last_error_message()
