@@ -264,7 +264,12 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
     auto it = g_tune.find(plan_key(p));
     const KernelEntry *e = nullptr;
     int order = 1;
-    if (it != g_tune.end() && parse_tuned(it->second, &e, &order)) {
+    // taken only if it computes this transform (a cache file is input: a
+    // line may pair a key with another length's or layout's kernel)
+    const KernelEntry *k0 = p->kernel;
+    if (it != g_tune.end() && parse_tuned(it->second, &e, &order) && e->kind == k0->kind &&
+        e->n == k0->n && e->col == k0->col && e->prec == k0->prec && order >= 0 &&
+        (total + e->per_block - 1) / e->per_block <= 0x7fffffff) {
       p->kernel = e;
       p->xcd_remap = order;
     }

@@ -89,3 +89,38 @@ def test_every_candidate_kernel(geom):
         plan.execute(0, d)
         assert rel_l2(to_host(d), ref) <= tol("f64" if prec == "f64" else "f32", n), name
     assert len(names) == k
+
+
+def test_cache_line_with_a_foreign_kernel_is_ignored(tmp_path):
+    """A cache file is input: a line that pairs a geometry with a kernel of
+    another length leaves the plan on its own default kernel (which then
+    gives the oracle's result)."""
+    import parafaft_amd as P
+    from parafaft_amd import backend as B
+    B.tune_cache_clear()
+    geoms = [(1024, 96), (2048, 48)]
+    defaults = {}
+    for n, batch in geoms:
+        scratch = to_dev(np.zeros(n * batch, np.complex64))
+        plan = P.Plan.c2c("f32", n, batch, batch, 1)
+        defaults[n] = plan.describe()[0]
+        plan.tune(1, scratch)
+    path = str(tmp_path / "pfx_tune.txt")
+    B.tune_cache_save(path)
+    lines = [l for l in open(path).read().splitlines() if l and not l.startswith("#")]
+    assert len(lines) == 2
+    (k1, v1), (k2, v2) = [l.split("\t") for l in lines]
+    with open(path, "w") as f:
+        f.write(f"{k1}\t{v2}\n{k2}\t{v1}\n")
+    B.tune_cache_clear()
+    B.tune_cache_load(path)
+    for n, batch in geoms:
+        plan = P.Plan.c2c("f32", n, batch, batch, 1)
+        assert plan.describe()[0] == defaults[n]
+        x = O.uniform_complex((n * batch,), seed=n).astype(np.complex64)
+        d = to_dev(x)
+        plan.execute(0, d)
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, batch, 1, -1)
+        assert rel_l2(to_host(d), ref) <= tol("f32", n)
+    B.tune_cache_clear()
