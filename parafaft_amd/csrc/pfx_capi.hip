@@ -471,7 +471,7 @@ static int to_segmap(const pfx_segmap *m, int length, int64_t total, SegMap *out
   out->nparts = m->nparts;
   int end = 0;
   for (int q = 0; q < m->nparts; ++q) {
-    if (m->start[q] != end || m->count[q] < 0)
+    if (m->start[q] != end || m->count[q] < 0 || m->count[q] > length - end)
       return fail(PFX_ERR_INVALID_ARG, "segment blocks must tile the axis in order");
     if (m->count[q] > 0 && !m->segments[q])
       return fail(PFX_ERR_INVALID_ARG, "segment %d has no address", q);
