@@ -577,9 +577,23 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   // each block as a run, with no per-element peer search.
   bool to_lds = false;
   if constexpr (SEG) to_lds = g.out_map != nullptr;
-  // post-twiddle: pairs (k, N-k) for k = 0..N/2
+  // post-twiddle: pairs (k, N-k) for k = j + i TPT <= N/2; the twiddle
+  // loads of all of them are issued before the first is used
   const T half = T(0.5);
-  for (int k = j; k <= N / 2; k += TPT) {
+  constexpr int KI = (N / 2 + TPT) / TPT;
+  V2<T> twk[KI], twm[KI];
+#pragma unroll
+  for (int i = 0; i < KI; ++i) {
+    const int k = j + i * TPT;
+    if (k <= N / 2) {
+      twk[i] = tw[k];
+      twm[i] = tw[N - k];
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < KI; ++i) {
+    const int k = j + i * TPT;
+    if (k > N / 2) continue;
     const int km = (N - k) % N;
     V2<T> A = lds[L::at(k, c)];
     V2<T> Bm = lds[L::at(km, c)];
@@ -587,7 +601,7 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
     {
       V2<T> B = conj<T>(Bm);
       V2<T> s = add<T>(A, B), d = sub<T>(A, B);
-      V2<T> wd = cmul<T>(tw[k], d);
+      V2<T> wd = cmul<T>(twk[i], d);
       V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
       if (g.use_scale) x = scl<T>(x, g.scale);
       if (to_lds) lds[L::at(k, c)] = x;
@@ -598,7 +612,7 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
       const int kk = N - k;
       V2<T> A2 = Bm, B2 = conj<T>(A);
       V2<T> s = add<T>(A2, B2), d = sub<T>(A2, B2);
-      V2<T> wd = cmul<T>(tw[kk], d);
+      V2<T> wd = cmul<T>(twm[i], d);
       V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
       if (g.use_scale) x = scl<T>(x, g.scale);
       if (to_lds) lds[L::at(kk, c)] = x;
@@ -654,10 +668,26 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
   const V2<T>* tw = static_cast<const V2<T>*>(a.tw);
-  for (int k = j; k <= N / 2; k += TPT) {
+  // every load of the pairs (k, N - k), k = j + i TPT <= N/2, is issued
+  // before the first is used, so they are in flight together
+  constexpr int KI = (N / 2 + TPT) / TPT;
+  V2<T> xk[KI], xm[KI];
+#pragma unroll
+  for (int i = 0; i < KI; ++i) {
+    const int k = j + i * TPT;
+    xk[i] = mk<T>(T(0), T(0));
+    xm[i] = xk[i];
+    if (g.valid && k <= N / 2) {
+      xk[i] = g.load(k);
+      xm[i] = g.load(N - k);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < KI; ++i) {
+    const int k = j + i * TPT;
+    if (k > N / 2) continue;
     const int km = N - k;
-    V2<T> Xk = mk<T>(T(0), T(0)), Xm = Xk;
-    if (g.valid) { Xk = g.load(k); Xm = g.load(km); }
+    V2<T> Xk = xk[i], Xm = xm[i];
     if (k == 0) { Xk.y = T(0); Xm.y = T(0); }
     // Z[k]
     {
