@@ -299,7 +299,7 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
   __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<float>* lds = B::data(smem);
   V2<float>* stw = B::tw(smem);
-  const V2<float>* tw = stage_twiddles<float, N, sizeof...(Rs)>(stw, a.ptw);
+  const V2<float>* tw = stage_twiddles<float, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
   const MfmaDft16<MODE> dft;
   int j, c;
   thread_coords<TPT, C, COL>(j, c);

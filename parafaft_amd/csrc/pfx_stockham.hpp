@@ -466,13 +466,26 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
   }
 };
 
-// Copy the per-pass twiddle table (N - 1 entries) into LDS; only plans with
-// more than one pass multiply by twiddles.
-template <typename T, int N, int NPASS>
+// Copy the per-pass twiddle table (N - 1 entries) into LDS with a
+// workgroup of NTHR threads; only plans with more than one pass multiply by
+// twiddles.  All of a thread's loads are issued before its first LDS store,
+// so they are in flight together.
+template <typename T, int N, int NPASS, int NTHR>
 __device__ __forceinline__ const V2<T>* stage_twiddles(V2<T>* stw, const void* ptw) {
   if constexpr (NPASS > 1) {
     const V2<T>* src = static_cast<const V2<T>*>(ptw);
-    for (int i = threadIdx.x; i < N - 1; i += blockDim.x) stw[i] = src[i];
+    constexpr int IT = (N - 1 + NTHR - 1) / NTHR;
+    V2<T> t[IT];
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = (int)threadIdx.x + i * NTHR;
+      if (e < N - 1) t[i] = src[e];
+    }
+#pragma unroll
+    for (int i = 0; i < IT; ++i) {
+      const int e = (int)threadIdx.x + i * NTHR;
+      if (e < N - 1) stw[e] = t[i];
+    }
     __syncthreads();
   }
   return stw;
@@ -517,7 +530,7 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
   __shared__ __align__(16) unsigned char smem[B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0)];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
-  const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
+  const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
   if (COL) tile = tile_order(tile, ntiles, a.xcd_remap);
@@ -555,7 +568,7 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
-  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
+  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = tile * C + c;
@@ -653,7 +666,7 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
   __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
-  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs)>(stw, a.ptw);
+  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = tile * C + c;
@@ -837,7 +850,7 @@ __device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t t
     j = threadIdx.x % TPT;
     c = threadIdx.x / TPT;
   } else {
-    tw = stage_twiddles<T, N, sizeof...(Rs)>(B::tw(smem), a.ptw);
+    tw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(B::tw(smem), a.ptw);
     c = threadIdx.x % C;
     j = threadIdx.x / C;
     tile = tile_order(tile, ntiles, a.xcd_remap);
