@@ -362,6 +362,17 @@ __device__ __forceinline__ void split_last_pass(V2<float>* v, const V2<float>* t
   }
 }
 
+// The same last pass with its twiddles already in registers
+// (t[g - 1] = W^{g jj}, loaded earlier).
+template <int R3>
+__device__ __forceinline__ void split_last_pass_tw(V2<float>* v, const V2<float>* t) {
+  if constexpr (R3 > 1) {
+#pragma unroll
+    for (int g = 1; g < R3; ++g) v[g] = cmul<float>(v[g], t[g - 1]);
+    bfly<float, R3>(v);
+  }
+}
+
 // Pass-2 twiddle W_256^{b m} for input m (1 for m = 0; the table index is
 // kept in range for every lane).
 __device__ __forceinline__ V2<float> split_pass2_twiddle(const V2<float>* tw, int m, int b) {
@@ -540,6 +551,22 @@ __device__ __forceinline__ void split_col_body(const StockhamArgs& a, V2<float>*
 #pragma unroll
     for (int r = 0; r < 4; ++r) lds[L::at(16 * (w + W * i) + 4 * q + r, c)] = y[r];
   }
+  // The pass-2 and pass-3 twiddles do not depend on the exchange: their
+  // loads are issued before the barrier, so they are in flight during it
+  // (the pass-1 values are dead by now, so they cost no extra registers).
+  constexpr int G3 = R3 > 1 ? R3 - 1 : 1;
+  V2<float> t2[P2][4], t3[P2][4][G3];
+#pragma unroll
+  for (int k = 0; k < P2; ++k) {
+    const int bb = w + W * k;
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      t2[k][r] = split_pass2_twiddle(tw, 4 * q + r, bb);
+      const int jj = bb + 16 * (4 * q + r);
+#pragma unroll
+      for (int g = 1; g < R3; ++g) t3[k][r][g - 1] = tw[g * 256 + jj - 1];
+    }
+  }
   __syncthreads();
   // pass 2 takes butterflies b + 16 k2, b = w + W k, k2 < R3, at slot k R3 + k2
 #pragma unroll
@@ -556,24 +583,20 @@ __device__ __forceinline__ void split_col_body(const StockhamArgs& a, V2<float>*
   for (int k = 0; k < P2; ++k) {
     const int bb = w + W * k;
     // pass 2 (span 16): twiddle W_256^{bb m}
-    V2<float> t2[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) t2[r] = split_pass2_twiddle(tw, 4 * q + r, bb);
 #pragma unroll
     for (int k2 = 0; k2 < R3; ++k2) {
       V2<float> x[4];
 #pragma unroll
-      for (int r = 0; r < 4; ++r) x[r] = cmul<float>(v[k * R3 + k2][r], t2[r]);
+      for (int r = 0; r < 4; ++r) x[r] = cmul<float>(v[k * R3 + k2][r], t2[k][r]);
       dft.apply(x, v[k * R3 + k2], split_sign(b, bb + 16 * k2));
     }
     // pass 3 in registers, then HBM at element jj + 256 k2
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-      const int jj = bb + 16 * (4 * q + r);
       V2<float> t[R3];
 #pragma unroll
       for (int k2 = 0; k2 < R3; ++k2) t[k2] = v[k * R3 + k2][r];
-      split_last_pass<R3>(t, tw, jj);
+      split_last_pass_tw<R3>(t, t3[k][r]);
       if (valid) {
 #pragma unroll
         for (int k2 = 0; k2 < R3; ++k2) {
