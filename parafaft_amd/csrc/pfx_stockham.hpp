@@ -466,28 +466,43 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
   }
 };
 
-// Copy the per-pass twiddle table (N - 1 entries) into LDS with a
-// workgroup of NTHR threads; only plans with more than one pass multiply by
-// twiddles.  All of a thread's loads are issued before its first LDS store,
-// so they are in flight together.
+// Copy of the per-pass twiddle table (N - 1 entries) into LDS by a
+// workgroup of NTHR threads, in two steps: fetch() issues this thread's
+// loads into registers, commit() stores them to LDS and ends in a barrier.
+// A kernel can issue its first-pass HBM loads in between: the twiddle loads
+// are older, so waiting for them leaves the data loads in flight.  Only
+// plans with more than one pass multiply by twiddles.
+template <typename T, int N, int NPASS, int NTHR> struct TwiddleStage {
+  static constexpr int IT = NPASS > 1 ? (N - 1 + NTHR - 1) / NTHR : 0;
+  V2<T> t[IT > 0 ? IT : 1];
+  __device__ __forceinline__ void fetch(const void* ptw) {
+    if constexpr (IT > 0) {
+      const V2<T>* src = static_cast<const V2<T>*>(ptw);
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        const int e = (int)threadIdx.x + i * NTHR;
+        if (e < N - 1) t[i] = src[e];
+      }
+    }
+  }
+  __device__ __forceinline__ void commit(V2<T>* stw) {
+    if constexpr (IT > 0) {
+#pragma unroll
+      for (int i = 0; i < IT; ++i) {
+        const int e = (int)threadIdx.x + i * NTHR;
+        if (e < N - 1) stw[e] = t[i];
+      }
+      __syncthreads();
+    }
+  }
+};
+
+// Both steps at once (kernels whose first pass is not issued separately).
 template <typename T, int N, int NPASS, int NTHR>
 __device__ __forceinline__ const V2<T>* stage_twiddles(V2<T>* stw, const void* ptw) {
-  if constexpr (NPASS > 1) {
-    const V2<T>* src = static_cast<const V2<T>*>(ptw);
-    constexpr int IT = (N - 1 + NTHR - 1) / NTHR;
-    V2<T> t[IT];
-#pragma unroll
-    for (int i = 0; i < IT; ++i) {
-      const int e = (int)threadIdx.x + i * NTHR;
-      if (e < N - 1) t[i] = src[e];
-    }
-#pragma unroll
-    for (int i = 0; i < IT; ++i) {
-      const int e = (int)threadIdx.x + i * NTHR;
-      if (e < N - 1) stw[e] = t[i];
-    }
-    __syncthreads();
-  }
+  TwiddleStage<T, N, NPASS, NTHR> ts;
+  ts.fetch(ptw);
+  ts.commit(stw);
   return stw;
 }
 
@@ -845,12 +860,20 @@ __device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t t
   T* lds = B::data(smem);
   const V2<T>* tw;
   int c, j;
+  // Column layout: the LDS twiddle copy is fetched first and committed after
+  // the first-pass loads are issued (the first pass uses no twiddles), except
+  // in 1024-thread tiles of 32 elements per thread: at their 128-VGPR limit
+  // the fetched entries spill, and those measured faster committing first.
+  constexpr bool EARLY = !(TPT * C == 1024 && E >= 32);
+  TwiddleStage<T, N, (ROWL ? 1 : (int)sizeof...(Rs)), TPT * C> tstage;
   if constexpr (ROWL) {
     tw = static_cast<const V2<T>*>(a.ptw);
     j = threadIdx.x % TPT;
     c = threadIdx.x / TPT;
   } else {
-    tw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(B::tw(smem), a.ptw);
+    tstage.fetch(a.ptw);
+    if constexpr (!EARLY) tstage.commit(B::tw(smem));
+    tw = B::tw(smem);
     c = threadIdx.x % C;
     j = threadIdx.x / C;
     tile = tile_order(tile, ntiles, a.xcd_remap);
@@ -878,6 +901,7 @@ __device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t t
       v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
     }
   }
+  if constexpr (!ROWL && EARLY) tstage.commit(B::tw(smem));
   WidePasses<T, N, TPT, WL, E, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
 }
 
