@@ -440,6 +440,16 @@ __device__ __forceinline__ void split_row_body(const StockhamArgs& a, V2<float>*
         v[i][r] = SW ? swp<float>(x) : x;
       }
   }
+  // pass-2 and pass-3 twiddles, loaded while the data loads are in flight
+  constexpr int G3 = R3 > 1 ? R3 - 1 : 1;
+  V2<float> t2[4], t3[4][G3];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    t2[r] = split_pass2_twiddle(tw, 4 * q + r, u);
+    const int jj = u + 16 * (4 * q + r);
+#pragma unroll
+    for (int g = 1; g < R3; ++g) t3[r][g - 1] = tw[g * 256 + jj - 1];
+  }
 #pragma unroll
   for (int i = 0; i < R3; ++i) {
     V2<float> y[4];
@@ -450,9 +460,6 @@ __device__ __forceinline__ void split_row_body(const StockhamArgs& a, V2<float>*
   }
   wave_lds_sync();
   // pass 2: butterfly 16 i + u (span 16), input m = 4 q + r at 16 i + u + NB m
-  V2<float> t2[4];
-#pragma unroll
-  for (int r = 0; r < 4; ++r) t2[r] = split_pass2_twiddle(tw, 4 * q + r, u);
 #pragma unroll
   for (int i = 0; i < R3; ++i) {
     V2<float> x[4];
@@ -469,11 +476,10 @@ __device__ __forceinline__ void split_row_body(const StockhamArgs& a, V2<float>*
   const int64_t s16 = 16 * st, s256 = 256 * st;
 #pragma unroll
   for (int r = 0; r < 4; ++r) {
-    const int jj = u + 16 * (4 * q + r);
     V2<float> t[R3];
 #pragma unroll
     for (int i = 0; i < R3; ++i) t[i] = v[i][r];
-    split_last_pass<R3>(t, tw, jj);
+    split_last_pass_tw<R3>(t, t3[r]);
 #pragma unroll
     for (int i = 0; i < R3; ++i) {
       const V2<float> x = scl<float>(t[i], sc);
