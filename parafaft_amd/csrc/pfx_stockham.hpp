@@ -408,6 +408,10 @@ __device__ __forceinline__ void init_global(GlobalPtr<T, SEG, TAB, NT>& g, const
   }
 }
 
+struct NoHook {
+  __device__ void operator()() const {}
+};
+
 // Run passes PI.. of the radix list.  The first pass may read HBM (FIRST_G),
 // the last may write HBM (LAST_G); everything else goes through LDS.
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G,
@@ -417,15 +421,21 @@ struct Passes;
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
           int Ns>
 struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns> {
-  template <class G> __device__ static __forceinline__ void run(int, int, V2<T>*, const V2<T>*, const G&) {}
+  template <class G, class H = NoHook>
+  __device__ static __forceinline__ void run(int, int, V2<T>*, const V2<T>*, const G&,
+                                             const H& = H()) {}
 };
 
 template <typename T, int N, int TPT, int C, bool COL, int TWS, bool FIRST_G, bool LAST_G, int PI,
           int Ns, int R, int... Rest>
 struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
   using L = LdsLayout<T, N, C, COL>;
-  template <class G>
-  __device__ static __forceinline__ void run(int j, int c, V2<T>* lds, const V2<T>* tw, const G& g) {
+  // hook() runs once in the first pass, after its loads and butterflies and
+  // before the barrier that precedes its LDS stores (a kernel commits its
+  // LDS twiddle copy there, which the second pass is the first to read)
+  template <class G, class H = NoHook>
+  __device__ static __forceinline__ void run(int j, int c, V2<T>* lds, const V2<T>* tw, const G& g,
+                                             const H& hook = H()) {
     constexpr bool IS_FIRST = (PI == 0);
     constexpr bool IS_LAST = (sizeof...(Rest) == 0);
     auto load = [&](int e) -> V2<T> {
@@ -438,8 +448,11 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
       }
     };
     struct StoreLds {
-      V2<T>* lds; int c;
-      __device__ void before() const { __syncthreads(); }
+      V2<T>* lds; int c; const H* hook;
+      __device__ void before() const {
+        if constexpr (IS_FIRST) (*hook)();
+        __syncthreads();
+      }
       __device__ void after() const { __syncthreads(); }
       __device__ void operator()(int e, V2<T> v) const { lds[L::at(e, c)] = v; }
     };
@@ -460,7 +473,7 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
     if constexpr (IS_LAST && LAST_G) {
       stockham_pass<T, N, R, Ns, TPT>(j, tw, load, StoreG{&g});
     } else {
-      stockham_pass<T, N, R, Ns, TPT>(j, tw, load, StoreLds{lds, c});
+      stockham_pass<T, N, R, Ns, TPT>(j, tw, load, StoreLds{lds, c, &hook});
     }
     Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI + 1, Ns * R, Rest...>::run(j, c, lds, tw, g);
   }
@@ -485,13 +498,19 @@ template <typename T, int N, int NPASS, int NTHR> struct TwiddleStage {
       }
     }
   }
-  __device__ __forceinline__ void commit(V2<T>* stw) {
+  // the LDS stores alone (the caller's next barrier publishes them)
+  __device__ __forceinline__ void put(V2<T>* stw) const {
     if constexpr (IT > 0) {
 #pragma unroll
       for (int i = 0; i < IT; ++i) {
         const int e = (int)threadIdx.x + i * NTHR;
         if (e < N - 1) stw[e] = t[i];
       }
+    }
+  }
+  __device__ __forceinline__ void commit(V2<T>* stw) {
+    if constexpr (IT > 0) {
+      put(stw);
       __syncthreads();
     }
   }
@@ -545,7 +564,11 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
   __shared__ __align__(16) unsigned char smem[B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0)];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
-  const V2<T>* tw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
+  // the LDS twiddle copy: fetched now, stored by the first pass (which uses
+  // none) before its barrier, so its loads overlap the first-pass loads
+  TwiddleStage<T, N, sizeof...(Rs), TPT * C> ts;
+  ts.fetch(a.ptw);
+  const V2<T>* tw = stw;
   int j, c;
   thread_coords<TPT, C, COL>(j, c);
   if (COL) tile = tile_order(tile, ntiles, a.xcd_remap);
@@ -561,7 +584,8 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
-  Passes<T, N, TPT, C, COL, 1, true, true, 0, 1, Rs...>::run(j, c, lds, tw, g);
+  Passes<T, N, TPT, C, COL, 1, true, true, 0, 1, Rs...>::run(j, c, lds, tw, g,
+                                                             [&] { ts.put(stw); });
 }
 
 template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
@@ -583,7 +607,9 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
-  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
+  TwiddleStage<T, N, sizeof...(Rs), TPT * C> ts;  // stored by the first pass
+  ts.fetch(a.ptw);
+  const V2<T>* ptw = stw;
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = tile * C + c;
@@ -598,7 +624,8 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
   const V2<T>* tw = static_cast<const V2<T>*>(a.tw);  // W_{2N}^i, i < 2N
-  Passes<T, N, TPT, C, false, 2, true, false, 0, 1, Rs...>::run(j, c, lds, ptw, g);
+  Passes<T, N, TPT, C, false, 2, true, false, 0, 1, Rs...>::run(j, c, lds, ptw, g,
+                                                                [&] { ts.put(stw); });
   // Segmented output: the post-twiddle writes X back into the LDS row (each
   // thread owns its pairs (k, N-k), so in place is safe; X[N] takes the
   // row's spare slot N), then one uniform loop over the peers' blocks stores
@@ -681,7 +708,9 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
   __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
-  const V2<T>* ptw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(stw, a.ptw);
+  TwiddleStage<T, N, sizeof...(Rs), TPT * C> ts;  // stored with the pre-twiddled row
+  ts.fetch(a.ptw);
+  const V2<T>* ptw = stw;
   int j, c;
   thread_coords<TPT, C, false>(j, c);
   const int64_t b = tile * C + c;
@@ -733,6 +762,7 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
       lds[L::at(km, c)] = swp<T>(z);
     }
   }
+  ts.put(stw);
   __syncthreads();
   Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, ptw, g);
 }
