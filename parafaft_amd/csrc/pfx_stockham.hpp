@@ -482,9 +482,10 @@ struct Passes<T, N, TPT, C, COL, TWS, FIRST_G, LAST_G, PI, Ns, R, Rest...> {
 // Copy of the per-pass twiddle table (N - 1 entries) into LDS by a
 // workgroup of NTHR threads, in two steps: fetch() issues this thread's
 // loads into registers, commit() stores them to LDS and ends in a barrier.
-// A kernel can issue its first-pass HBM loads in between: the twiddle loads
-// are older, so waiting for them leaves the data loads in flight.  Only
-// plans with more than one pass multiply by twiddles.
+// A kernel issues its first-pass HBM loads in between, so the twiddle loads
+// and the data loads are in flight together instead of one after the other
+// with a barrier between.  Only plans with more than one pass multiply by
+// twiddles.
 template <typename T, int N, int NPASS, int NTHR> struct TwiddleStage {
   static constexpr int IT = NPASS > 1 ? (N - 1 + NTHR - 1) / NTHR : 0;
   V2<T> t[IT > 0 ? IT : 1];
