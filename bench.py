@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "3D FFT GFLOP/s + HBM TB/s, 1024^3 fp32, rel-L2≤2e-6 vs FP64; split vs plain"
 HBM_PEAK_GBS = 8000.0
+ACC_BOUND = 2e-6          # rel-L2 against the FP64 path, every FP32 form
 
 
 def log(msg):
@@ -613,7 +614,18 @@ def main():
             log(f"[bench] {mode}: {sms:.3f} ms/step ({ms_per_step / sms:.3f}x plain)")
             L.pfx_ndplan_destroy(nds.value)
         out["split_vs_plain"] = split
+    # the metric's accuracy bound (rel-L2 <= 2e-6 against FP64) for every
+    # FP32 form measured; a run that breaks it exits non-zero after printing
+    ok = None
+    if acc:
+        forms = [acc] + [v["accuracy"] for v in out.get("split_vs_plain", {}).values()
+                         if v.get("accuracy")]
+        ok = all(f["rel_l2_vs_fp64"] <= ACC_BOUND for f in forms)
+    out["accuracy_ok"] = ok
     print(json.dumps(out), flush=True)
+    if ok is False:
+        log(f"[bench] accuracy bound rel-L2 <= {ACC_BOUND} vs FP64 exceeded")
+        sys.exit(3)
 
 
 if __name__ == "__main__":

@@ -187,7 +187,12 @@ int pfx_plan_use_candidate(pfx_plan plan, int index);
  * Replaces: FFTW wisdom export / import (backend/fftw3/fft_backend_fftw.hpp:666-694).
  */
 int pfx_tune_cache_save(const char *path);
+/* Lines with a key not in the cache's key format or a kernel this build does
+ * not have are dropped; the load still succeeds with the rest, and
+ * pfx_last_error_message() then names the file and the dropped count. */
 int pfx_tune_cache_load(const char *path);
+/* Lines the last pfx_tune_cache_load dropped. */
+int pfx_tune_cache_dropped(void);
 int pfx_tune_cache_clear(void);
 /* Number of entries in the tuning cache. */
 int pfx_tune_cache_size(void);

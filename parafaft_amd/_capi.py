@@ -65,6 +65,7 @@ SIGNATURES = [
     ("pfx_plan_use_candidate", _i, [_vp, _i]),
     ("pfx_tune_cache_save", _i, [C.c_char_p]),
     ("pfx_tune_cache_load", _i, [C.c_char_p]),
+    ("pfx_tune_cache_dropped", _i, []),
     ("pfx_tune_cache_clear", _i, []),
     ("pfx_tune_cache_size", _i, []),
     ("pfx_malloc", _i, [_vpp, _sz]),

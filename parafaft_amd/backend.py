@@ -53,8 +53,11 @@ def tune_cache_save(path: str):
     A.check(A.load().pfx_tune_cache_save(path.encode()), "pfx_tune_cache_save")
 
 
-def tune_cache_load(path: str):
+def tune_cache_load(path: str) -> int:
+    """Loads a tuning cache file; returns the number of lines dropped (a key
+    not in the cache's format, or a kernel this build does not have)."""
     A.check(A.load().pfx_tune_cache_load(path.encode()), "pfx_tune_cache_load")
+    return A.load().pfx_tune_cache_dropped()
 
 
 def tune_cache_clear():

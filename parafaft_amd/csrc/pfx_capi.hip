@@ -148,6 +148,7 @@ namespace {
 // description and workgroup tile identify it uniquely in the registry).
 std::mutex g_tune_mu;
 std::map<std::string, std::string> g_tune;
+int g_tune_dropped = 0;  // lines the last pfx_tune_cache_load dropped
 
 std::string entry_id(const KernelEntry *k) {
   return std::to_string(k->prec) + "|" + std::to_string(k->kind) + "|" + std::to_string(k->n) +
@@ -175,6 +176,37 @@ std::string plan_key(const pfx_plan_t *p) {
            (long long)p->length, (long long)p->inner, (long long)p->outer, (long long)p->stride,
            (long long)p->dist, (long long)p->dist_outer);
   return buf;
+}
+
+// True when the kernel `e` computes the transform the plan's default
+// kernel `k0` computes (same kind, length, layout and arithmetic) and one
+// launch of it fits the grid limit.  A tuning-cache value is input (a
+// loaded file may pair a key with another length's or layout's kernel), so
+// every use of one goes through this check.
+bool kernel_fits_plan(const pfx_plan_t *p, const KernelEntry *k0, const KernelEntry *e,
+                      int order) {
+  if (!k0 || !e || order < 0) return false;
+  if (e->kind != k0->kind || e->n != k0->n || e->col != k0->col || e->prec != k0->prec)
+    return false;
+  return (p->inner * p->outer + e->per_block - 1) / e->per_block <= 0x7fffffff;
+}
+
+// The cache entry of the plan's key, if there is one and it passes
+// kernel_fits_plan against `k0`.  Caller holds g_tune_mu.
+bool cached_choice(const pfx_plan_t *p, const KernelEntry *k0, const KernelEntry **e,
+                   int *order) {
+  auto it = g_tune.find(plan_key(p));
+  if (it == g_tune.end() || !parse_tuned(it->second, e, order)) return false;
+  return kernel_fits_plan(p, k0, *e, *order);
+}
+
+// Valid-looking plan key: the fields plan_key writes, in its order.
+bool is_plan_key(const char *s) {
+  int pr, kd;
+  long long n, i, o, st, d, dout;
+  char tail;
+  return sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld%c", &pr, &kd, &n, &i, &o, &st,
+                &d, &dout, &tail) == 8;
 }
 
 }  // namespace
@@ -261,15 +293,9 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
   if (lds_n > 0 && !p->kernel) p->kernel = find_kernel(bp, kk, lds_n, col);
   if (p->kernel) {  // a tuned choice for this geometry wins over the table
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find(plan_key(p));
     const KernelEntry *e = nullptr;
     int order = 1;
-    // taken only if it computes this transform (a cache file is input: a
-    // line may pair a key with another length's or layout's kernel)
-    const KernelEntry *k0 = p->kernel;
-    if (it != g_tune.end() && parse_tuned(it->second, &e, &order) && e->kind == k0->kind &&
-        e->n == k0->n && e->col == k0->col && e->prec == k0->prec && order >= 0 &&
-        (total + e->per_block - 1) / e->per_block <= 0x7fffffff) {
+    if (cached_choice(p, p->kernel, &e, &order)) {
       p->kernel = e;
       p->xcd_remap = order;
     }
@@ -566,11 +592,13 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
   if (!scratch) return fail(PFX_ERR_INVALID_ARG, "tuning needs a scratch buffer");
   const std::string key = plan_key(p);
   {
+    // a cached choice is taken only if it computes this plan's transform
+    // (the plan's current kernel has the same kind, length, layout and
+    // arithmetic as every candidate); otherwise it is measured afresh
     std::lock_guard<std::mutex> lock(g_tune_mu);
-    auto it = g_tune.find(key);
     const KernelEntry *e = nullptr;
     int order = 1;
-    if (it != g_tune.end() && parse_tuned(it->second, &e, &order)) {
+    if (cached_choice(p, p->kernel, &e, &order)) {
       PFX_HIP(set_kernel(p, e), "tuned kernel");
       p->xcd_remap = order;
       return PFX_SUCCESS;
@@ -660,12 +688,23 @@ int pfx_tune_cache_load(const char *path) {
     while (!val.empty() && (val.back() == '\n' || val.back() == '\r')) val.pop_back();
     const KernelEntry *ke = nullptr;
     int order = 1;
-    if (!parse_tuned(val, &ke, &order)) { ++bad; continue; }  // kernel not in this build
+    // a key not in plan_key's format, or a kernel not in this build
+    if (!is_plan_key(line) || !parse_tuned(val, &ke, &order) || order < 0) { ++bad; continue; }
     g_tune[line] = val;
   }
   fclose(f);
-  (void)bad;
+  g_tune_dropped = bad;
+  if (bad > 0) {
+    // the load succeeds with the valid lines; the message says what was dropped
+    fail(PFX_SUCCESS, "tuning cache %s: %d line(s) dropped (bad key or kernel not in this build)",
+         path, bad);
+  }
   return PFX_SUCCESS;
+}
+
+int pfx_tune_cache_dropped(void) {
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  return g_tune_dropped;
 }
 
 int pfx_tune_cache_clear(void) {

@@ -33,3 +33,34 @@ def tol(prec: str, n: int) -> float:
     if prec == "f64":
         return 1e-13
     return 1e-6 * max(1.0, np.log2(max(n, 2)))
+
+
+# Which lengths the library must accept: every product of a radix plan in
+# pfx_plans.inc has tiled kernels (C2C, and R2C/C2R of twice the length), and
+# any length up to the direct-DFT limit (2n complex values in 64 KiB of LDS)
+# runs the fallback.  A plan outside both must be refused; every other plan
+# must be created (a refusal there is a failure, not a skip).
+NAIVE_MAX = {"f32": 4096, "f64": 2048}
+
+
+def tiled_lengths():
+    import os
+    import re
+    path = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "parafaft_amd", "csrc", "pfx_plans.inc")
+    out = set()
+    for m in re.finditer(r"^PFX_PLAN\(\s*\d+\s*,([\d,\s]+)\)", open(path).read(), re.M):
+        n = 1
+        for r in m.group(1).split(","):
+            n *= int(r)
+        out.add(n)
+    return out
+
+
+def c2c_supported(prec: str, n: int) -> bool:
+    return n in tiled_lengths() or n <= NAIVE_MAX["f64" if prec == "f64" else "f32"]
+
+
+def real_supported(prec: str, n: int) -> bool:
+    return (n % 2 == 0 and n // 2 in tiled_lengths()) or \
+        n <= NAIVE_MAX["f64" if prec == "f64" else "f32"]

@@ -107,3 +107,37 @@ def test_batch_product_overflow_rejected():
     assert rc == 1
     assert h.value is None
     assert b"overflows" in L.pfx_last_error_message()
+
+
+@pytest.mark.parametrize("prec,n", [(0, 8192), (0, 4097), (1, 3072), (1, 4096 + 2)])
+def test_lengths_without_a_kernel_are_refused_on_the_host(prec, n):
+    """C2C lengths with no tiled plan beyond the direct-DFT limit (4096
+    complex<float>, 2048 complex<double>) are refused before any device
+    work, with PFX_ERR_UNSUPPORTED; the GPU tests hold the complementary
+    rule (every other length must plan)."""
+    import sys
+    sys.path.insert(0, ROOT)
+    from tests.gpu_util import c2c_supported
+    import parafaft_amd as P
+    L = P.load_library()
+    assert not c2c_supported("f64" if prec else "f32", n)
+    h = C.c_void_p()
+    assert L.pfx_plan_create_c2c(C.byref(h), prec, n, 4, 1, n) == 2
+    assert h.value is None
+    assert b"fallback" in L.pfx_last_error_message()
+
+
+def test_tune_cache_load_counts_dropped_lines(tmp_path):
+    """Malformed keys and kernels this build lacks are dropped and counted
+    (host-only: no plan is created)."""
+    from parafaft_amd import backend as B
+    path = tmp_path / "cache.txt"
+    path.write_text("# header\n"
+                    "garbage\tvalue\n"
+                    "p0 k0 n16 i4 o1 s1 d16 do0\t0|0|16|0|1|nope|order=1\n"
+                    "no tab\n")
+    B.tune_cache_clear()
+    assert B.tune_cache_load(str(path)) == 3
+    assert B.tune_cache_size() == 0
+    import parafaft_amd as P
+    assert b"3 line(s) dropped" in P.load_library().pfx_last_error_message()

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 BUILD = os.path.join(HERE, "cpp", "_build")
 
 
-EXES = ["test_mi355_backend", "test_mi355_nd", "test_pfxvector"]
+EXES = ["test_mi355_backend", "test_mi355_nd", "test_pfxvector", "test_mi355_mpi"]
 
 
 @pytest.mark.parametrize("exe", EXES + [f"asan/{e}" for e in EXES])

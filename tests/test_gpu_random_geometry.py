@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_util import rel_l2, to_dev, to_host, tol
+from tests.gpu_util import c2c_supported, real_supported, rel_l2, to_dev, to_host, tol
 
 pytestmark = pytest.mark.gpu
 
@@ -64,10 +64,11 @@ def test_random_c2c_geometry(case):
     ref = x.astype(np.complex128).copy()
     O.c2c_batched(ref, n, batch, stride, dist, -1 if direction == 0 else +1)
     ref *= scale
-    try:
-        plan = P.Plan.c2c(prec, n, batch, stride, dist)
-    except P.PfxError as e:  # no tiled plan and past the direct-DFT limit
-        pytest.skip(f"unsupported: {e}")
+    if not c2c_supported(prec, n):   # no tiled plan and past the direct-DFT limit
+        with pytest.raises(P.PfxError):
+            P.Plan.c2c(prec, n, batch, stride, dist)
+        return
+    plan = P.Plan.c2c(prec, n, batch, stride, dist)
     d = to_dev(x)
     plan.execute(direction, d, scale=scale)
     got = to_host(d)
@@ -96,11 +97,12 @@ def test_random_real_geometry(case):
     buf = O.uniform(batch * dist, seed=9000 + i).reshape(batch, dist).astype(rdt)
     ref = buf.astype(np.float64).copy()
     O.r2c_inplace(ref, n, batch, 1, dist)
-    try:
-        r2c = P.Plan.r2c(prec, n, batch, 1, dist)
-        c2r = P.Plan.c2r(prec, n, batch, 1, dist)
-    except P.PfxError as e:
-        pytest.skip(f"unsupported: {e}")
+    if not real_supported(prec, n):
+        with pytest.raises(P.PfxError):
+            P.Plan.r2c(prec, n, batch, 1, dist)
+        return
+    r2c = P.Plan.r2c(prec, n, batch, 1, dist)
+    c2r = P.Plan.c2r(prec, n, batch, 1, dist)
     d = to_dev(buf)
     r2c.execute(0, d)
     got = to_host(d)

@@ -113,14 +113,45 @@ def test_cache_line_with_a_foreign_kernel_is_ignored(tmp_path):
     with open(path, "w") as f:
         f.write(f"{k1}\t{v2}\n{k2}\t{v1}\n")
     B.tune_cache_clear()
-    B.tune_cache_load(path)
+    assert B.tune_cache_load(path) == 0
     for n, batch in geoms:
         plan = P.Plan.c2c("f32", n, batch, batch, 1)
         assert plan.describe()[0] == defaults[n]
         x = O.uniform_complex((n * batch,), seed=n).astype(np.complex64)
-        d = to_dev(x)
-        plan.execute(0, d)
         ref = x.astype(np.complex128).copy()
         O.c2c_batched(ref, n, batch, batch, 1, -1)
+        d = to_dev(x)
+        plan.execute(0, d)
         assert rel_l2(to_host(d), ref) <= tol("f32", n)
+        # a Measure tune must not take the foreign line either: it measures
+        # afresh and keeps a kernel of this plan's own length
+        scratch = to_dev(np.zeros(n * batch, np.complex64))
+        plan.tune(1, scratch)
+        assert f"_n{n}_" in plan.describe()[0], plan.describe()
+        d = to_dev(x)
+        plan.execute(0, d)
+        assert rel_l2(to_host(d), ref) <= tol("f32", n)
+    B.tune_cache_clear()
+
+
+def test_cache_load_counts_dropped_lines(tmp_path):
+    """Lines whose key is not in the cache's format, or whose kernel this
+    build lacks, are dropped and counted; valid lines still load."""
+    import parafaft_amd as P
+    from parafaft_amd import backend as B
+    B.tune_cache_clear()
+    scratch = to_dev(np.zeros(256 * 64, np.complex64))
+    P.Plan.c2c("f32", 256, 64, 64, 1).tune(1, scratch)
+    path = str(tmp_path / "pfx_tune.txt")
+    B.tune_cache_save(path)
+    good = [l for l in open(path).read().splitlines() if l and not l.startswith("#")]
+    assert len(good) == 1
+    key, val = good[0].split("\t")
+    with open(path, "a") as f:
+        f.write(f"not a key\t{val}\n")                 # bad key
+        f.write(f"{key.replace('n256', 'n512')}\t0|0|999|1|1|nope|order=1\n")  # no such kernel
+        f.write("no tab on this line\n")
+    B.tune_cache_clear()
+    assert B.tune_cache_load(path) == 3
+    assert B.tune_cache_size() == 1
     B.tune_cache_clear()
