@@ -144,6 +144,7 @@ def bench_pencil(args, torch, A):
 
 
 def bench_batch(args, torch, A):
+    import parafaft_amd as P
     """512 uneven transforms (seeded draw: 40% {48..128}^3 C2C, 30% {1024..4096}^2
     C2C, 20% {80,120,160}^3 R2C, 10% 32^4 C2C), complex<float>: one launch per
     transform stage vs grouped launches vs the grouped schedule replayed as a
@@ -192,6 +193,15 @@ def bench_batch(args, torch, A):
 
     timed("one_launch_per_transform_stage", lambda: b.execute_each(A.PFX_FORWARD, sp))
     timed("grouped", lambda: b.execute(A.PFX_FORWARD, sp))
+    pinfo = None
+    try:
+        pinfo = b.persistent_info(A.PFX_FORWARD)
+    except P.PfxError as e:
+        log(f"[bench] no persistent schedule: {e}")
+    if pinfo:
+        timed("persistent", lambda: b.execute_persistent(A.PFX_FORWARD, sp))
+        if b.persistent_failed(sp):
+            raise RuntimeError("persistent launch stopped at its spin limit")
     graph = Graph(sp, lambda: b.execute(A.PFX_FORWARD, sp))
     timed("grouped_graph", lambda: graph.launch(sp))
     graph_each = Graph(sp, lambda: b.execute_each(A.PFX_FORWARD, sp))
@@ -258,6 +268,10 @@ def bench_batch(args, torch, A):
         "results": res,
         "speedup_grouped_vs_each": round(res["one_launch_per_transform_stage"]["ms"] /
                                          res["grouped"]["ms"], 3),
+        "persistent": ({"jobs": pinfo[0], "tickets": pinfo[1], "workgroups": pinfo[2],
+                        "speedup_vs_each": round(res["one_launch_per_transform_stage"]["ms"] /
+                                                 res["persistent"]["ms"], 3)}
+                       if pinfo else None),
         "speedup_graph_vs_each": round(res["one_launch_per_transform_stage"]["ms"] /
                                        res["grouped_graph"]["ms"], 3),
         "speedup_graph_vs_reference_driver": (

@@ -355,6 +355,39 @@ int pfx_batch_execute(pfx_batch batch, int direction, pfx_stream stream);
 /* All transforms, one launch per transform stage (the ungrouped baseline). */
 int pfx_batch_execute_each(pfx_batch batch, int direction, pfx_stream stream);
 
+/*
+ * All transforms in ONE persistent launch (plus a small kernel that clears
+ * its counters): a grid of one 1024-thread workgroup per CU claims tickets
+ * (one tile of one transform stage) from per-XCD queues -- every transform
+ * runs on one XCD, so its stages share an L2 -- and stage s of a transform
+ * starts only when every tile of its stage s - 1 has completed (device-side
+ * counters).  Complex<float> batches whose stage lengths all
+ * have radix plans; otherwise PFX_ERR_UNSUPPORTED (use pfx_batch_execute).
+ * Where a stage's kernel and the persistent body share their radices (the
+ * default plans), the result is bit-identical to pfx_batch_execute.  A wait
+ * that exceeds the spin limit ends the launch early and is reported by
+ * pfx_batch_persistent_status.  No counterpart in the reference (its
+ * nearest analogue is one ParaFaFT object per shape issuing its own
+ * launches, parafaft_c2c.hpp:539-602).
+ */
+int pfx_batch_execute_persistent(pfx_batch batch, int direction, pfx_stream stream);
+
+/* Synchronises `stream`, then sets *failed to 0 if the last persistent
+ * launch of `batch` completed, 1 if it stopped at its spin limit, or 2 if a
+ * ticket queue was left with work (no workgroup ran on that XCD); after a
+ * nonzero status the results are incomplete. */
+int pfx_batch_persistent_status(pfx_batch batch, pfx_stream stream, int *failed);
+
+/* Polls of a completion counter that may find it short before a persistent
+ * launch gives up (default 2^21, seconds of waiting; a legitimate wait takes
+ * microseconds).  0: give up at the first poll that finds it short. */
+int pfx_batch_set_spin_limit(pfx_batch batch, int polls);
+
+/* Jobs (transform stages), tickets (tiles) and workgroups of the persistent
+ * schedule for one direction; PFX_ERR_UNSUPPORTED if the batch has none. */
+int pfx_batch_persistent_info(pfx_batch batch, int direction, int *jobs, int *tickets,
+                              int *workgroups);
+
 /* Launch counts of the two schedules for one direction. */
 int pfx_batch_launches(pfx_batch batch, int direction, int *grouped, int *each);
 

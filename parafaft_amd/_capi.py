@@ -91,6 +91,10 @@ SIGNATURES = [
     ("pfx_batch_tune", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute_each", _i, [_vp, _i, _vp]),
+    ("pfx_batch_execute_persistent", _i, [_vp, _i, _vp]),
+    ("pfx_batch_persistent_status", _i, [_vp, _vp, _ip]),
+    ("pfx_batch_set_spin_limit", _i, [_vp, _i]),
+    ("pfx_batch_persistent_info", _i, [_vp, _i, _ip, _ip, _ip]),
     ("pfx_batch_launches", _i, [_vp, _i, _ip, _ip]),
     ("pfx_batch_destroy", _i, [_vp]),
     ("pfx_graph_begin_capture", _i, [_vp]),

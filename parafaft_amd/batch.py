@@ -42,6 +42,32 @@ class Batch:
         A.check(A.load().pfx_batch_execute_each(self.handle, direction, ptr(stream)),
                 "pfx_batch_execute_each")
 
+    def execute_persistent(self, direction, stream=None):
+        """All transforms in one persistent launch (tickets from per-XCD
+        queues, device-side stage dependencies)."""
+        A.check(A.load().pfx_batch_execute_persistent(self.handle, direction, ptr(stream)),
+                "pfx_batch_execute_persistent")
+
+    def persistent_failed(self, stream=None) -> bool:
+        """Synchronises `stream`; True if the last persistent launch stopped
+        at its spin limit."""
+        f = C.c_int()
+        A.check(A.load().pfx_batch_persistent_status(self.handle, ptr(stream), C.byref(f)),
+                "pfx_batch_persistent_status")
+        return bool(f.value)
+
+    def set_spin_limit(self, polls: int):
+        A.check(A.load().pfx_batch_set_spin_limit(self.handle, int(polls)),
+                "pfx_batch_set_spin_limit")
+
+    def persistent_info(self, direction):
+        """(jobs, tickets, workgroups) of the persistent schedule."""
+        j, t, w = C.c_int(), C.c_int(), C.c_int()
+        A.check(A.load().pfx_batch_persistent_info(self.handle, direction, C.byref(j),
+                                                   C.byref(t), C.byref(w)),
+                "pfx_batch_persistent_info")
+        return j.value, t.value, w.value
+
     def launches(self, direction):
         g, e = C.c_int(), C.c_int()
         A.check(A.load().pfx_batch_launches(self.handle, direction, C.byref(g), C.byref(e)),

@@ -13,6 +13,7 @@
 
 #include "../../include/pfx_capi.h"
 #include "pfx_internal.hpp"
+#include "pfx_persist.hpp"
 
 using namespace pfx;
 
@@ -1262,11 +1263,36 @@ StockhamArgs job_args(pfx_plan p, void *data, int direction) {
 
 }  // namespace
 
+namespace {
+
+// The persistent schedule of one direction: one job per transform stage, in
+// an order where every job's dependency (the same transform's previous
+// stage) comes first; tiles of each job split over the per-XCD queues.
+struct Persist {
+  bool ok = false;
+  std::string why;  // why there is no persistent schedule
+  std::vector<PJob> jobs;
+  std::vector<StockhamArgs> args;
+  std::vector<int> qstart;  // kPersistQueues rows of njobs + 1 offsets
+  int tickets = 0;
+  PJob *d_jobs = nullptr;
+  StockhamArgs *d_args = nullptr;
+  int *d_qstart = nullptr;
+};
+
+}  // namespace
+
 struct pfx_batch_t {
   int precision = PFX_F32;
   std::vector<pfx_ndplan> plans;
   std::vector<void *> data;
   std::vector<Level> levels[2];  // [direction]: every chunk's stage levels, in order
+  Persist persist[2];            // [direction]
+  int *d_ctl = nullptr;          // control words + per-job counters (persistent launch)
+  std::map<std::string, void *> persist_tw;  // pass tables for bodies with other radices
+  int spin_limit = 1 << 21;
+  int grid = 0;
+  int nqueues = 1;
 };
 
 struct pfx_graph_t {
@@ -1389,6 +1415,228 @@ int build_levels(pfx_batch b) {
   return PFX_SUCCESS;
 }
 
+void free_persist(pfx_batch b) {
+  for (Persist &P : b->persist) {
+    if (P.d_jobs) (void)hipFree(P.d_jobs);
+    if (P.d_args) (void)hipFree(P.d_args);
+    if (P.d_qstart) (void)hipFree(P.d_qstart);
+    P = Persist();
+  }
+  if (b->d_ctl) (void)hipFree(b->d_ctl);
+  b->d_ctl = nullptr;
+}
+
+bool same_radices(const Radices &x, const Radices &y) {
+  if (x.n != y.n) return false;
+  for (int i = 0; i < x.n; ++i)
+    if (x.r[i] != y.r[i]) return false;
+  return true;
+}
+
+// The persistent body computing stage plan `pl`: same kind, length and
+// layout, with the radices of the plan's kernel when a body has them (then
+// the arithmetic, and the result, are the plan's bit for bit).  -1: none.
+int persist_body_for(const pfx_plan_t *pl) {
+  if (!pl->kernel) return -1;  // lengths without a radix plan (direct DFT)
+  int nb = 0;
+  const PersistBody *B = persist_bodies(&nb);
+  const int kk = pl->kind == PFX_C2C ? KIND_C2C : (pl->kind == PFX_R2C ? KIND_R2C : KIND_C2R);
+  int col;
+  if (pl->kind != PFX_C2C || pl->stride == 1) col = 0;
+  else if (pl->dist == 1) col = 1;
+  else return -1;
+  if (pl->kind != PFX_C2C && pl->stride != 1) return -1;
+  int any = -1;
+  for (int i = 0; i < nb; ++i) {
+    if (B[i].kind != kk || B[i].n != pl->kernel->n || B[i].col != col) continue;
+    if (same_radices(B[i].rad, pl->kernel->rad)) return i;
+    if (any < 0) any = i;
+  }
+  return any;
+}
+
+// Per-pass twiddle table of a body's radices (the plan's own when they match).
+int body_pass_table(pfx_batch b, const pfx_plan_t *pl, const PersistBody &body,
+                    const void **out) {
+  if (same_radices(body.rad, pl->kernel->rad)) {
+    *out = pl->ptw;
+    return PFX_SUCCESS;
+  }
+  std::string key = std::to_string(body.n) + ":" + body.name;
+  auto it = b->persist_tw.find(key);
+  if (it == b->persist_tw.end()) {
+    void *t = nullptr;
+    hipError_t e = hipMalloc(&t, (size_t)body.n * 8);
+    if (e == hipSuccess) e = make_pass_twiddles(0, body.rad, t, nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) {
+      if (t) (void)hipFree(t);
+      return hip_fail(e, "persistent pass table");
+    }
+    it = b->persist_tw.emplace(key, t).first;
+  }
+  *out = it->second;
+  return PFX_SUCCESS;
+}
+
+// Builds both directions' persistent schedules.  Every transform goes to
+// one queue (one XCD), largest first to the queue with the fewest bytes so
+// far.  Within a queue the transforms below PFX_BATCH_SMALL_MB (16) MiB come
+// first, stage level by stage level; the larger ones follow pipelined,
+// largest first: stage k of the r-th of them sits in wave r + k L
+// (L = PFX_PERSIST_LAG, 1 by default), so a stage's tickets come a wave
+// after those of the stage it waits for, while the next transform's first
+// stage keeps the XCD busy.
+int build_persist(pfx_batch b) {
+  free_persist(b);
+  const int nq = persist_queues();
+  b->nqueues = nq;
+  for (int dir = 0; dir < 2; ++dir) {
+    Persist &P = b->persist[dir];
+    P.ok = false;
+    if (b->precision != PFX_F32) {
+      P.why = "the persistent kernel is built for complex<float> batches";
+      continue;
+    }
+    b->grid = persist_grid();
+    if (b->grid <= 0) {
+      P.why = "no device occupancy for the persistent kernel";
+      continue;
+    }
+    const int count = (int)b->plans.size();
+    const int64_t small_bytes = env_mb("PFX_BATCH_SMALL_MB", 16) << 20;
+    const int lag = (int)env_mb("PFX_PERSIST_LAG", 1);
+    // transform -> queue, largest first to the least loaded queue
+    std::vector<int> by_size(count);
+    for (int i = 0; i < count; ++i) by_size[i] = i;
+    std::stable_sort(by_size.begin(), by_size.end(), [&](int x, int y) {
+      return footprint(b->plans[x]) > footprint(b->plans[y]);
+    });
+    std::vector<int64_t> load(nq, 0);
+    std::vector<std::vector<int>> small(nq), big(nq);  // in size order
+    for (int i : by_size) {
+      const int q = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[q] += footprint(b->plans[i]);
+      (footprint(b->plans[i]) >= small_bytes ? big : small)[q].push_back(i);
+    }
+    std::vector<std::pair<int, int>> order;  // (transform, stage step k), queue by queue
+    std::vector<int> order_q;
+    for (int q = 0; q < nq; ++q) {
+      int maxd = 0;
+      for (int i : small[q]) maxd = std::max(maxd, b->plans[i]->ndim);
+      for (int k = 0; k < maxd; ++k)
+        for (int i : small[q])
+          if (k < b->plans[i]->ndim) {
+            order.push_back({i, k});
+            order_q.push_back(q);
+          }
+      struct W { int w, k, r; };
+      std::vector<W> waves;
+      for (int r = 0; r < (int)big[q].size(); ++r)
+        for (int k = 0; k < b->plans[big[q][r]]->ndim; ++k) waves.push_back({r + k * lag, k, r});
+      std::stable_sort(waves.begin(), waves.end(), [](const W &x, const W &y) {
+        return x.w != y.w ? x.w < y.w : x.k < y.k;
+      });
+      for (const W &w : waves) {
+        order.push_back({big[q][w.r], w.k});
+        order_q.push_back(q);
+      }
+    }
+    int nb = 0;
+    const PersistBody *B = persist_bodies(&nb);
+    std::vector<int> last(count, -1);  // job of each transform's latest stage
+    std::vector<int> job_q;            // queue of every job
+    bool ok = true;
+    for (size_t o = 0; o < order.size(); ++o) {
+      const int i = order[o].first, k = order[o].second;
+      pfx_ndplan nd = b->plans[i];
+      const int D = nd->ndim;
+      const int st = dir == PFX_FORWARD ? k : D - 1 - k;
+      pfx_plan pl = nd->stages[st];
+      if (nd->kind == PFX_R2C && st == 0 && dir == PFX_BACKWARD) pl = nd->stages[D];
+      const int body = persist_body_for(pl);
+      if (body < 0) {
+        P.why = "transform " + std::to_string(i) + " has a stage (length " +
+                std::to_string(pl->length) + ") without a persistent kernel body";
+        ok = false;
+        break;
+      }
+      StockhamArgs a = job_args(pl, b->data[i], dir);
+      a.xcd_remap = 0;  // a queue's tiles run in order on one XCD
+      int rc = body_pass_table(b, pl, B[body], &a.ptw);
+      if (rc != PFX_SUCCESS) return rc;
+      if (a.total == 0) continue;
+      const int64_t tiles = (a.total + B[body].per_block - 1) / B[body].per_block;
+      if (tiles > (1 << 30)) {
+        P.why = "a stage has too many tiles for the ticket queues";
+        ok = false;
+        break;
+      }
+      PJob J;
+      J.body = body;
+      J.ntiles = (int)tiles;
+      J.dep = last[i];
+      J.need = J.dep >= 0 ? P.jobs[J.dep].ntiles : 0;
+      last[i] = (int)P.jobs.size();
+      P.jobs.push_back(J);
+      P.args.push_back(a);
+      job_q.push_back(order_q[o]);
+    }
+    if (!ok) {
+      P.jobs.clear();
+      P.args.clear();
+      continue;
+    }
+    const int nj = (int)P.jobs.size();
+    // every wait is on an earlier job of the same queue, for all of its
+    // tiles (the progress argument of the persistent kernel rests on this)
+    for (int j = 0; j < nj; ++j) {
+      const PJob &J = P.jobs[j];
+      if (J.body < 0 || J.body >= nb || J.ntiles < 1 || J.dep >= j ||
+          (J.dep >= 0 && (J.need != P.jobs[J.dep].ntiles || job_q[J.dep] != job_q[j])))
+        return fail(PFX_ERR_INTERNAL, "persistent schedule: job %d breaks the ticket order", j);
+    }
+    P.qstart.assign((size_t)kPersistQueues * (nj + 1), 0);
+    int64_t tickets = 0;
+    for (int x = 0; x < kPersistQueues; ++x) {
+      int64_t pos = 0;
+      for (int j = 0; j < nj; ++j) {
+        P.qstart[(size_t)x * (nj + 1) + j] = (int)pos;
+        if (job_q[j] == x) pos += P.jobs[j].ntiles;
+      }
+      P.qstart[(size_t)x * (nj + 1) + nj] = (int)pos;
+      tickets += pos;
+      if (pos > 0x7fffffff) return fail(PFX_ERR_UNSUPPORTED, "ticket queue exceeds 2^31 entries");
+    }
+    P.tickets = (int)tickets;
+    if (nj > 0) {
+      hipError_t e = hipMalloc(&P.d_jobs, nj * sizeof(PJob));
+      if (e == hipSuccess) e = hipMalloc(&P.d_args, nj * sizeof(StockhamArgs));
+      if (e == hipSuccess) e = hipMalloc(&P.d_qstart, P.qstart.size() * sizeof(int));
+      if (e == hipSuccess)
+        e = hipMemcpy(P.d_jobs, P.jobs.data(), nj * sizeof(PJob), hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(P.d_args, P.args.data(), nj * sizeof(StockhamArgs), hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(P.d_qstart, P.qstart.data(), P.qstart.size() * sizeof(int),
+                      hipMemcpyHostToDevice);
+      if (e != hipSuccess) return hip_fail(e, "persistent job tables");
+    }
+    P.ok = true;
+  }
+  size_t ctl = kCtlInts;
+  ctl += std::max(b->persist[0].jobs.size(), b->persist[1].jobs.size());
+  hipError_t e = hipMalloc(&b->d_ctl, ctl * sizeof(int));
+  if (e != hipSuccess) return hip_fail(e, "persistent control words");
+  return PFX_SUCCESS;
+}
+
+int build_schedules(pfx_batch b) {
+  int rc = build_levels(b);
+  if (rc == PFX_SUCCESS) rc = build_persist(b);
+  return rc;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1396,6 +1644,8 @@ extern "C" {
 int pfx_batch_destroy(pfx_batch b) {
   if (!b) return PFX_SUCCESS;
   free_levels(b);
+  free_persist(b);
+  for (auto &kv : b->persist_tw) (void)hipFree(kv.second);
   for (pfx_ndplan p : b->plans) pfx_ndplan_destroy(p);
   delete b;
   return PFX_SUCCESS;
@@ -1424,7 +1674,7 @@ int pfx_batch_create(pfx_batch *out, int precision, int count, const pfx_batch_d
     b->plans.push_back(nd);
     b->data.push_back(d.data);
   }
-  int rc = build_levels(b);
+  int rc = build_schedules(b);
   if (rc != PFX_SUCCESS) {
     std::string msg = g_last_error;
     pfx_batch_destroy(b);
@@ -1443,7 +1693,7 @@ int pfx_batch_tune(pfx_batch b, int flag, pfx_stream stream) {
   }
   hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
   if (e != hipSuccess) return hip_fail(e, "planning");
-  return build_levels(b);
+  return build_schedules(b);
 }
 
 int pfx_batch_execute(pfx_batch b, int direction, pfx_stream stream) {
@@ -1472,6 +1722,56 @@ int pfx_batch_execute_each(pfx_batch b, int direction, pfx_stream stream) {
     int rc = pfx_ndplan_execute(b->plans[i], direction, b->data[i], 1.0, stream);
     if (rc != PFX_SUCCESS) return rc;
   }
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_execute_persistent(pfx_batch b, int direction, pfx_stream stream) {
+  if (!b) return fail(PFX_ERR_INVALID_ARG, "batch is null");
+  if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "bad direction");
+  Persist &P = b->persist[direction];
+  if (!P.ok) return fail(PFX_ERR_UNSUPPORTED, "no persistent schedule: %s", P.why.c_str());
+  if (P.jobs.empty()) return PFX_SUCCESS;
+  PersistArgs a;
+  a.jobs = P.d_jobs;
+  a.args = P.d_args;
+  a.qstart = P.d_qstart;
+  a.njobs = (int)P.jobs.size();
+  a.ctl = b->d_ctl;
+  a.done = b->d_ctl + kCtlInts;
+  a.spin_limit = b->spin_limit;
+  a.nqueues = b->nqueues;
+  PFX_HIP(launch_persist(a, b->grid, static_cast<hipStream_t>(stream)), "persistent launch");
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_persistent_status(pfx_batch b, pfx_stream stream, int *failed) {
+  if (!b || !failed) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  *failed = 0;
+  if (!b->d_ctl) return PFX_SUCCESS;
+  PFX_HIP(hipStreamSynchronize(static_cast<hipStream_t>(stream)), "hipStreamSynchronize");
+  int err = 0;
+  PFX_HIP(hipMemcpy(&err, b->d_ctl + kCtlError, sizeof(int), hipMemcpyDeviceToHost),
+          "persistent status");
+  *failed = err;
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_set_spin_limit(pfx_batch b, int polls) {
+  if (!b) return fail(PFX_ERR_INVALID_ARG, "batch is null");
+  if (polls < 0) return fail(PFX_ERR_INVALID_ARG, "spin limit must not be negative");
+  b->spin_limit = polls;
+  return PFX_SUCCESS;
+}
+
+int pfx_batch_persistent_info(pfx_batch b, int direction, int *jobs, int *tickets,
+                              int *workgroups) {
+  if (!b || direction < 0 || direction > 1) return fail(PFX_ERR_INVALID_ARG, "bad argument");
+  const Persist &P = b->persist[direction];
+  if (!P.ok) return fail(PFX_ERR_UNSUPPORTED, "no persistent schedule: %s", P.why.c_str());
+  if (jobs) *jobs = (int)P.jobs.size();
+  if (tickets) *tickets = P.tickets;
+  if (workgroups) *workgroups = b->grid;
   return PFX_SUCCESS;
 }
 

@@ -36,6 +36,31 @@ enum Kind { KIND_C2C = 0, KIND_R2C = 1, KIND_C2R = 2 };
 
 template <int R, int... Rest> constexpr int first_of() { return R; }
 
+// Element i of a table in device (global) memory, read through a
+// global-address-space pointer (a pointer that came from a job table in
+// memory would otherwise be read with a generic, flat, load).
+template <typename T> __device__ __forceinline__ V2<T> gload(const V2<T>* p, int64_t i) {
+  typedef T E2 __attribute__((ext_vector_type(2)));
+  const E2 x = *((const __attribute__((address_space(1))) E2*)(p + i));
+  return mk<T>(x.x, x.y);
+}
+
+// Thread index of the calling lane in its workgroup, as the kernel bodies
+// read it.  A kernel that runs many bodies in a loop (the persistent kernel)
+// defines PFX_OPAQUE_TID before including this header: the index then comes
+// out of an empty asm statement at every use, so the compiler cannot hoist
+// the bodies' index and LDS-address arithmetic out of the loop, where all of
+// it would stay live across every body and spill.
+__device__ __forceinline__ int lane_tid() {
+#ifdef PFX_OPAQUE_TID
+  int t = threadIdx.x;
+  asm volatile("" : "+v"(t));
+  return t;
+#else
+  return threadIdx.x;
+#endif
+}
+
 // One axis of a row-major block split into per-peer segments, in the order
 // the pencil exchange packs it (parafaft_common.hpp:576-590): element
 // (lead, e, trail) of the block, with e on the split axis, lives at
@@ -248,23 +273,34 @@ template <typename T, bool SEG = false, bool TAB = false, bool NT = false> struc
   bool valid;
   T scale;
   bool use_scale;
+  // The data is device (global) memory.  Its pointers reach a kernel either
+  // as kernel arguments (the compiler then knows they are global) or from a
+  // job table in memory (grouped and persistent kernels), where it cannot
+  // tell and would issue generic (flat) accesses: those count against both
+  // the vector-memory and the LDS wait counters, so every LDS access between
+  // them forces a full wait.  Accesses go through global-address-space
+  // two-element vectors instead.
   typedef T E2 __attribute__((ext_vector_type(2)));
+  using GE2 = __attribute__((address_space(1))) E2;
   __device__ __forceinline__ V2<T> load(int e) const {
+    const GE2* q = (const GE2*)(pin + (int64_t)e * stride);
     if constexpr (NT) {
-      const E2 x = __builtin_nontemporal_load(reinterpret_cast<const E2*>(pin + (int64_t)e * stride));
+      const E2 x = __builtin_nontemporal_load(q);
       return mk<T>(x.x, x.y);
     } else {
-      return pin[(int64_t)e * stride];
+      const E2 x = *q;
+      return mk<T>(x.x, x.y);
     }
   }
   __device__ __forceinline__ void store(int e, V2<T> v) const {
+    GE2* q = (GE2*)(p + (int64_t)e * stride);
+    E2 x;
+    x.x = v.x;
+    x.y = v.y;
     if constexpr (NT) {
-      E2 x;
-      x.x = v.x;
-      x.y = v.y;
-      __builtin_nontemporal_store(x, reinterpret_cast<E2*>(p + (int64_t)e * stride));
+      __builtin_nontemporal_store(x, q);
     } else {
-      p[(int64_t)e * stride] = v;
+      *q = x;
     }
   }
 };
@@ -494,8 +530,8 @@ template <typename T, int N, int NPASS, int NTHR> struct TwiddleStage {
       const V2<T>* src = static_cast<const V2<T>*>(ptw);
 #pragma unroll
       for (int i = 0; i < IT; ++i) {
-        const int e = (int)threadIdx.x + i * NTHR;
-        if (e < N - 1) t[i] = src[e];
+        const int e = lane_tid() + i * NTHR;
+        if (e < N - 1) t[i] = gload<T>(src, e);
       }
     }
   }
@@ -504,7 +540,7 @@ template <typename T, int N, int NPASS, int NTHR> struct TwiddleStage {
     if constexpr (IT > 0) {
 #pragma unroll
       for (int i = 0; i < IT; ++i) {
-        const int e = (int)threadIdx.x + i * NTHR;
+        const int e = lane_tid() + i * NTHR;
         if (e < N - 1) stw[e] = t[i];
       }
     }
@@ -545,7 +581,7 @@ template <typename T, typename D, int TW, int DATA> struct LdsBlock {
 
 template <int TPT, int C, bool COL>
 __device__ __forceinline__ void thread_coords(int& j, int& c) {
-  const int tid = threadIdx.x;
+  const int tid = lane_tid();
   if constexpr (COL) { c = tid % C; j = tid / C; }
   else { j = tid % TPT; c = tid / TPT; }
 }
@@ -556,13 +592,23 @@ __device__ __forceinline__ void thread_coords(int& j, int& c) {
 // Kernel bodies take the workgroup's tile id and the tile count of its
 // launch (or of its job in a grouped launch), so the single-plan kernels and
 // the grouped multi-transform kernels share them.
+// LDS bytes of one c2c_body workgroup (tile, twiddle table, segment tables).
+template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs> struct C2CLds {
+  static constexpr int N = (Rs * ... * 1);
+  using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), LdsLayout<T, N, C, COL>::size>;
+  static constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
+  static constexpr int bytes = B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0);
+};
+
+// The body on an LDS block of C2CLds<...>::bytes the caller provides (a
+// kernel's own static block, or the shared block of the persistent kernel).
 template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
-__device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+__device__ __forceinline__ void c2c_body_in(const StockhamArgs& a, int64_t tile, int64_t ntiles,
+                                            unsigned char* smem) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, COL>;
   using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
   constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
-  __shared__ __align__(16) unsigned char smem[B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0)];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
   // the LDS twiddle copy: fetched now, stored by the first pass (which uses
@@ -590,6 +636,12 @@ __device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, in
 }
 
 template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
+__device__ __forceinline__ void c2c_body(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+  __shared__ __align__(16) unsigned char smem[C2CLds<T, TPT, C, COL, SEG, Rs...>::bytes];
+  c2c_body_in<T, TPT, C, COL, SEG, Rs...>(a, tile, ntiles, smem);
+}
+
+template <typename T, int TPT, int C, bool COL, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) c2c_kernel(StockhamArgs a) {
   c2c_body<T, TPT, C, COL, SEG, Rs...>(a, blockIdx.x, gridDim.x);
 }
@@ -600,12 +652,19 @@ __global__ void __launch_bounds__(TPT * C) c2c_kernel(StockhamArgs a) {
 // k = 0..N, written as N+1 complex values over the same padded row.
 // Row layout only (stride 1).  a.dist is in complex units (= padded dist / 2).
 // ---------------------------------------------------------------------------
+// LDS bytes of one r2c_body / c2r_body workgroup.
+template <typename T, int TPT, int C, int... Rs> struct RealLds {
+  static constexpr int N = (Rs * ... * 1);
+  static constexpr int bytes =
+      LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), LdsLayout<T, N, C, false>::size>::bytes;
+};
+
 template <typename T, int TPT, int C, bool SEG, int... Rs>
-__device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
+__device__ __forceinline__ void r2c_body_in(const StockhamArgs& a, int64_t tile,
+                                            unsigned char* smem) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
   using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
-  __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
   TwiddleStage<T, N, sizeof...(Rs), TPT * C> ts;  // stored by the first pass
@@ -642,8 +701,8 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
   for (int i = 0; i < KI; ++i) {
     const int k = j + i * TPT;
     if (k <= N / 2) {
-      twk[i] = tw[k];
-      twm[i] = tw[N - k];
+      twk[i] = gload<T>(tw, k);
+      twm[i] = gload<T>(tw, N - k);
     }
   }
 #pragma unroll
@@ -692,6 +751,12 @@ __device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
 }
 
 template <typename T, int TPT, int C, bool SEG, int... Rs>
+__device__ __forceinline__ void r2c_body(const StockhamArgs& a, int64_t tile) {
+  __shared__ __align__(16) unsigned char smem[RealLds<T, TPT, C, Rs...>::bytes];
+  r2c_body_in<T, TPT, C, SEG, Rs...>(a, tile, smem);
+}
+
+template <typename T, int TPT, int C, bool SEG, int... Rs>
 __global__ void __launch_bounds__(TPT * C) r2c_kernel(StockhamArgs a) {
   r2c_body<T, TPT, C, SEG, Rs...>(a, blockIdx.x);
 }
@@ -702,11 +767,11 @@ __global__ void __launch_bounds__(TPT * C) r2c_kernel(StockhamArgs a) {
 // gives z[n] = x[2n] + i x[2n+1], stored as the N complex slots of the row.
 // ---------------------------------------------------------------------------
 template <typename T, int TPT, int C, bool SEG, int... Rs>
-__device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
+__device__ __forceinline__ void c2r_body_in(const StockhamArgs& a, int64_t tile,
+                                            unsigned char* smem) {
   constexpr int N = (Rs * ... * 1);
   using L = LdsLayout<T, N, C, false>;
   using B = LdsBlock<T, V2<T>, (sizeof...(Rs) > 1 ? N : 1), L::size>;
-  __shared__ __align__(16) unsigned char smem[B::bytes];
   V2<T>* lds = B::data(smem);
   V2<T>* stw = B::tw(smem);
   TwiddleStage<T, N, sizeof...(Rs), TPT * C> ts;  // stored with the pre-twiddled row
@@ -751,14 +816,14 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
     {
       V2<T> B = conj<T>(Xm);
       V2<T> s = add<T>(Xk, B), d = sub<T>(Xk, B);
-      V2<T> wd = cmulc<T>(d, tw[k]);        // W^-k d
+      V2<T> wd = cmulc<T>(d, gload<T>(tw, k));  // W^-k d
       V2<T> z = mk<T>(s.x - wd.y, s.y + wd.x);  // s + i wd
       lds[L::at(k, c)] = swp<T>(z);
     }
     if (k != 0 && 2 * k != N) {
       V2<T> B = conj<T>(Xk);
       V2<T> s = add<T>(Xm, B), d = sub<T>(Xm, B);
-      V2<T> wd = cmulc<T>(d, tw[km]);
+      V2<T> wd = cmulc<T>(d, gload<T>(tw, km));
       V2<T> z = mk<T>(s.x - wd.y, s.y + wd.x);
       lds[L::at(km, c)] = swp<T>(z);
     }
@@ -766,6 +831,12 @@ __device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
   ts.put(stw);
   __syncthreads();
   Passes<T, N, TPT, C, false, 2, false, true, 0, 1, Rs...>::run(j, c, lds, ptw, g);
+}
+
+template <typename T, int TPT, int C, bool SEG, int... Rs>
+__device__ __forceinline__ void c2r_body(const StockhamArgs& a, int64_t tile) {
+  __shared__ __align__(16) unsigned char smem[RealLds<T, TPT, C, Rs...>::bytes];
+  c2r_body_in<T, TPT, C, SEG, Rs...>(a, tile, smem);
 }
 
 template <typename T, int TPT, int C, int... Rs>
@@ -791,7 +862,10 @@ namespace pfx {
 //    workgroup's LDS at N * C scalars and several workgroups per CU.
 // Requires every radix to divide E (all passes are exact).
 // ---------------------------------------------------------------------------
-template <int N, int C, bool ROWL> struct WideLayout {
+template <int N, int C, bool ROWL, bool GTW = ROWL> struct WideLayout {
+  // twiddles from the global table (the row layout's, and batch-contiguous
+  // tiles that leave the LDS to the data); otherwise staged in LDS
+  static constexpr bool global_tw = GTW;
   // row: one pad slot every 16 entries keeps the stride-R writes of a pass
   // with Ns = 1 off a single bank
   static constexpr int NP = N + N / 16 + 1;
@@ -824,7 +898,13 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
       if constexpr (Ns > 1) {
         const int kk = jj % Ns;
 #pragma unroll
-        for (int m = 1; m < R; ++m) v[q * R + m] = cmul<T>(v[q * R + m], tw[m * Ns + kk - 1]);
+        for (int m = 1; m < R; ++m) {
+          const int ti = m * Ns + kk - 1;
+          V2<T> w;
+          if constexpr (WL::global_tw) w = gload<T>(tw, ti);
+          else w = tw[ti];
+          v[q * R + m] = cmul<T>(v[q * R + m], w);
+        }
       }
       bfly<T, R>(v + q * R);
     }
@@ -877,17 +957,28 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
   }
 };
 
-template <typename T, int TPT, int C, bool ROWL, bool SEG, bool NT, int... Rs>
-__device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+// LDS bytes of one wide-body workgroup.
+template <typename T, int TPT, int C, bool ROWL, bool SEG, bool GTW, int... Rs> struct WideLds {
+  static constexpr int N = (Rs * ... * 1);
+  using B = LdsBlock<T, T, (!GTW && sizeof...(Rs) > 1 ? N : 0), WideLayout<N, C, ROWL, GTW>::size>;
+  static constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
+  static constexpr int bytes = B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0);
+};
+
+// GTW: twiddles read from the global table (always for the row layout; a
+// batch-contiguous tile can use it to leave the whole LDS block to its data).
+template <typename T, int TPT, int C, bool ROWL, bool SEG, bool NT, bool GTW, int... Rs>
+__device__ __forceinline__ void c2c_wide_body_in(const StockhamArgs& a, int64_t tile,
+                                                 int64_t ntiles, unsigned char* smem) {
+  static_assert(GTW || !ROWL, "the row layout reads twiddles from the global table");
   constexpr int N = (Rs * ... * 1);
   constexpr int E = N / TPT;
   constexpr int R0 = first_of<Rs...>();
   constexpr int Q0 = E / R0;
   constexpr int NB0 = N / R0;
-  using WL = WideLayout<N, C, ROWL>;
-  using B = LdsBlock<T, T, (!ROWL && sizeof...(Rs) > 1 ? N : 0), WL::size>;
+  using WL = WideLayout<N, C, ROWL, GTW>;
+  using B = LdsBlock<T, T, (!GTW && sizeof...(Rs) > 1 ? N : 0), WL::size>;
   constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
-  __shared__ __align__(16) unsigned char smem[B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0)];
   T* lds = B::data(smem);
   const V2<T>* tw;
   int c, j;
@@ -896,17 +987,20 @@ __device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t t
   // in 1024-thread tiles of 32 elements per thread: at their 128-VGPR limit
   // the fetched entries spill, and those measured faster committing first.
   constexpr bool EARLY = !(TPT * C == 1024 && E >= 32);
-  TwiddleStage<T, N, (ROWL ? 1 : (int)sizeof...(Rs)), TPT * C> tstage;
-  if constexpr (ROWL) {
+  TwiddleStage<T, N, (GTW ? 1 : (int)sizeof...(Rs)), TPT * C> tstage;
+  if constexpr (GTW) {
     tw = static_cast<const V2<T>*>(a.ptw);
-    j = threadIdx.x % TPT;
-    c = threadIdx.x / TPT;
   } else {
     tstage.fetch(a.ptw);
     if constexpr (!EARLY) tstage.commit(B::tw(smem));
     tw = B::tw(smem);
-    c = threadIdx.x % C;
-    j = threadIdx.x / C;
+  }
+  if constexpr (ROWL) {
+    j = lane_tid() % TPT;
+    c = lane_tid() / TPT;
+  } else {
+    c = lane_tid() % C;
+    j = lane_tid() / C;
     tile = tile_order(tile, ntiles, a.xcd_remap);
   }
   const int64_t b = tile * C + c;
@@ -932,8 +1026,14 @@ __device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t t
       v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
     }
   }
-  if constexpr (!ROWL && EARLY) tstage.commit(B::tw(smem));
+  if constexpr (!GTW && EARLY) tstage.commit(B::tw(smem));
   WidePasses<T, N, TPT, WL, E, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
+}
+
+template <typename T, int TPT, int C, bool ROWL, bool SEG, bool NT, int... Rs>
+__device__ __forceinline__ void c2c_wide_body_x(const StockhamArgs& a, int64_t tile, int64_t ntiles) {
+  __shared__ __align__(16) unsigned char smem[WideLds<T, TPT, C, ROWL, SEG, ROWL, Rs...>::bytes];
+  c2c_wide_body_in<T, TPT, C, ROWL, SEG, NT, ROWL, Rs...>(a, tile, ntiles, smem);
 }
 
 template <typename T, int TPT, int C, bool ROWL, bool SEG, int... Rs>

@@ -177,3 +177,34 @@ def test_each_graph_bit_identical_to_grouped(wl):
         stream.synchronize()
         assert t.equal(t.view_as_real(x), t.view_as_real(grouped)), rep
     del graph, x, b, grouped
+
+
+def test_persistent_single_launch(wl):
+    """The whole workload in one persistent launch: bit-identical to the
+    grouped launches (the persistent bodies use the default plans' radices),
+    so every transform also meets the FP64 bound checked above; the backward
+    launch likewise."""
+    t = torch()
+    x = wl["x0"].clone()
+    b = _batch(wl, x)
+    jobs, tickets, wgs = b.persistent_info(FORWARD)
+    assert jobs == sum(len(s) for _, s in wl["items"])
+    print(f"persistent: {jobs} jobs, {tickets} tickets, {wgs} workgroups", flush=True)
+    b.execute(FORWARD)
+    t.cuda.synchronize()
+    grouped = x.clone()
+    x.copy_(wl["x0"])
+    t.cuda.synchronize()
+    b.execute_persistent(FORWARD)
+    assert not b.persistent_failed()
+    assert t.equal(t.view_as_real(x), t.view_as_real(grouped))
+    _check_vs_fp64(wl, x)
+    b.execute(BACKWARD)
+    t.cuda.synchronize()
+    grouped_back = x.clone()
+    x.copy_(grouped)
+    t.cuda.synchronize()
+    b.execute_persistent(BACKWARD)
+    assert not b.persistent_failed()
+    assert t.equal(t.view_as_real(x), t.view_as_real(grouped_back))
+    del x, b, grouped, grouped_back
