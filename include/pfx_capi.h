@@ -158,6 +158,26 @@ int pfx_execute_segmented(pfx_plan plan, int direction, void *data, const pfx_se
 int pfx_plan_destroy(pfx_plan plan);
 
 /*
+ * One C2C transform with no plan and no allocation: the plan
+ * pfx_plan_create_c2c(precision, length, batch, stride, dist) would build
+ * (its default kernel, not a tuned one) is set up on the stack, its twiddle
+ * tables are generated into a fixed device scratch array of the library (a
+ * kernel, on `stream`), and the transform runs as pfx_execute_scaled would:
+ * same kernel, same tables, so the same bits.  Nothing is allocated on the
+ * device or the host and no call synchronises, so callers that cannot call
+ * hipMalloc (code under HIP graph capture, tight serving loops) can use it.
+ * Limits: PFX_F32 / PFX_F64, length <= 4096 (larger lengths are refused with
+ * PFX_ERR_INVALID_ARG; the scratch holds 2 x 4096 complex<double>).  Every
+ * call on one GPU uses that GPU's one scratch array, so calls that overlap
+ * on one GPU (different streams, or threads) race; calls in one stream are
+ * ordered.  No counterpart in the reference: it is for callers that cannot
+ * allocate.
+ */
+int pfx_execute_c2c_noalloc(int precision, int64_t length, int64_t batch, int64_t stride,
+                            int64_t dist, int direction, void *data, double scale,
+                            pfx_stream stream);
+
+/*
  * Planning effort (the FFTPlanFlag of backend/fft_backend.hpp:39-43).
  * PFX_PLAN_ESTIMATE: a static priority table picks the kernel.
  * PFX_PLAN_MEASURE / PFX_PLAN_PATIENT: pfx_plan_tune times every candidate

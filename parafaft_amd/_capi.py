@@ -59,6 +59,7 @@ SIGNATURES = [
     ("pfx_execute_r2c_oop", _i, [_vp, _vp, _i64, _vp, _d, _vp]),
     ("pfx_execute_c2r_oop", _i, [_vp, _vp, _vp, _i64, _d, _vp]),
     ("pfx_plan_destroy", _i, [_vp]),
+    ("pfx_execute_c2c_noalloc", _i, [_i, _i64, _i64, _i64, _i64, _i, _vp, _d, _vp]),
     ("pfx_plan_describe", _i, [_vp, C.c_char_p, _sz, _ip, _ip]),
     ("pfx_plan_tune", _i, [_vp, _i, _vp, _vp]),
     ("pfx_plan_num_candidates", _i, [_vp, _ip]),

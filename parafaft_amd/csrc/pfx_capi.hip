@@ -751,6 +751,103 @@ int pfx_plan_describe(pfx_plan p, char *name, size_t name_len, int *per_block, i
   return PFX_SUCCESS;
 }
 
+// ------------------------------------------------------------- no-alloc C2C
+
+}  // extern "C"
+
+namespace {
+
+// Device scratch of pfx_execute_c2c_noalloc: the length-n table W_n^i and
+// the per-pass table of the kernel, n <= kNoAllocMaxLength complex<double>
+// (128 KiB).  Each GPU the library runs on holds its own copy until the
+// process ends.  In host code `scratch` is a stand-in for the symbol, not a
+// device address: the device address comes from hipGetSymbolAddress, once
+// per device, and only that address is passed to kernels.
+constexpr int kNoAllocMaxLength = 4096;
+constexpr size_t kNoAllocScratchBytes = 2 * (size_t)kNoAllocMaxLength * 16;
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices];  // device address of `scratch` per device (null: not looked up)
+
+// The kernel registry is built when the library is loaded, so no call of
+// the no-alloc entry point allocates host memory for it.
+const Registry &g_registry_at_load = registry();
+
+int scratch_of_current_device(void **out) {
+  int dev = 0;
+  PFX_HIP(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(PFX_ERR_UNSUPPORTED, "device %d beyond the %d the scratch table holds", dev,
+                kMaxDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *ptr = nullptr;
+    PFX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr) return fail(PFX_ERR_HIP, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return PFX_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pfx_execute_c2c_noalloc(int precision, int64_t length, int64_t batch, int64_t stride,
+                            int64_t dist, int direction, void *data, double scale,
+                            pfx_stream stream) {
+  if (precision != PFX_F32 && precision != PFX_F64)
+    return fail(PFX_ERR_UNSUPPORTED, "the no-alloc transform takes PFX_F32 or PFX_F64 (got %d)",
+                precision);
+  if (length <= 0) return fail(PFX_ERR_INVALID_ARG, "length must be positive (got %lld)", (long long)length);
+  if (length > kNoAllocMaxLength)
+    return fail(PFX_ERR_INVALID_ARG,
+                "length %lld exceeds the no-alloc scratch maximum of %d (use a plan)",
+                (long long)length, kNoAllocMaxLength);
+  if (batch < 0) return fail(PFX_ERR_INVALID_ARG, "batch must be non-negative");
+  if (stride <= 0) return fail(PFX_ERR_INVALID_ARG, "stride must be positive (got %lld)", (long long)stride);
+  if (direction != PFX_FORWARD && direction != PFX_BACKWARD)
+    return fail(PFX_ERR_INVALID_ARG, "direction must be PFX_FORWARD or PFX_BACKWARD");
+  if (batch == 0) return PFX_SUCCESS;
+  if (!data) return fail(PFX_ERR_INVALID_ARG, "data is null");
+  // the plan pfx_plan_create_c2c would build (its default kernel), on the
+  // stack, with both twiddle tables in the device scratch
+  pfx_plan_t p;
+  p.precision = precision;
+  p.kind = PFX_C2C;
+  p.length = length;
+  p.inner = batch;
+  p.outer = 1;
+  p.stride = stride;
+  p.dist = dist;
+  p.dist_outer = 0;
+  p.xcd_remap = xcd_remap_mode();
+  const int bp = base_prec(precision);
+  const bool col = dist == 1 && stride > 1;
+  p.kernel = find_kernel(bp, KIND_C2C, length, col);
+  if (!p.kernel && length > naive_max_length(bp))
+    return fail(PFX_ERR_UNSUPPORTED,
+                "length %lld has no tiled kernel and exceeds the direct-DFT fallback limit of %d",
+                (long long)length, naive_max_length(bp));
+  const int64_t per = p.kernel ? p.kernel->per_block : 1;
+  if ((batch + per - 1) / per > 0x7fffffff)
+    return fail(PFX_ERR_UNSUPPORTED, "batch too large for one launch");
+  void *base = nullptr;
+  int rc = scratch_of_current_device(&base);
+  if (rc != PFX_SUCCESS) return rc;
+  const size_t esz = bp ? 16 : 8;
+  p.tw = base;
+  p.ptw = static_cast<unsigned char *>(base) + (size_t)length * esz;
+  p.tw_len = length;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  PFX_HIP(make_twiddles(bp, length, p.tw, s), "twiddle table");
+  if (p.kernel) PFX_HIP(make_pass_twiddles(bp, p.kernel->rad, p.ptw, s), "pass table");
+  return execute_impl(&p, direction, data, scale, stream, nullptr, 0);
+}
+
 // ------------------------------------------------------------------ memory
 
 int pfx_malloc(void **ptr, size_t bytes) {
