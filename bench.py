@@ -36,30 +36,72 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n: int, frac_log2: int = 3):
-    """Time the FP64 oracle (restated FFTWBackend<double> batched plans) on a
-    bounded sample of the workload: for each of the three axes, 1/2^frac_log2
-    of that axis' n*n length-n transforms, laid out with that axis' stride
-    pattern in an [n/2^frac_log2 ... ] block.  Returns GFLOP/s."""
+def cpu_baseline(n: int, frac_log2: int = 3, reps: int = 5):
+    """The CPU column, timed with the reference benchmark's protocol
+    (benchmark/bench_c2c.cpp:61-76: one warm-up forward+backward, then timed
+    forward+backward pairs; here `reps` of them, median and mean +- sd) for
+    both precisions of FFTWBackend (batch-parallel serial 1D plans per axis,
+    fft_backend_fftw.hpp:387-424, threads as at :768-808).  FFTW is not
+    installed, so the plans are the oracle's restated ones: its FP64 FFT
+    core, and the same core built in float.  The sample is a bounded part of
+    the 1024^3 workload: for each of the three axes, 1/2^frac_log2 of that
+    axis' n*n length-n transforms with that axis' stride pattern; the times
+    are scaled by 2^frac_log2 to one whole transform."""
+    import ctypes.util
+    import statistics
     import numpy as np
     from oracle import oracle as O
+    fftw = {lib: ctypes.util.find_library(lib) for lib in ("fftw3", "fftw3f")}
     m = (n * n) >> frac_log2          # transforms per axis in the sample
-    lead = m // n                     # slices
-    buf = O.uniform_complex((m * n,), seed=12345)
+    lead = m // n                     # slices of the middle axis
     threads = O.max_threads()
-    t0 = time.perf_counter()
-    O.c2c_batched(buf, n, m, 1, n, -1, threads)                  # axis 2: rows
-    for i in range(lead):                                        # axis 1: [lead, n, n]
-        sl = buf[i * n * n:(i + 1) * n * n]
-        O.c2c_batched(sl, n, n, n, 1, -1, threads)
-    O.c2c_batched(buf, n, m, m, 1, -1, threads)                  # axis 0: [n, lead, n]
-    dt = time.perf_counter() - t0
-    flops = 3 * m * 5.0 * n * math.log2(n)
-    return {"value": round(flops / dt / 1e9, 3), "unit": "GFLOP/s", "cores": threads,
+    flops_full = 5.0 * n ** 3 * math.log2(n ** 3)
+    scale = 1 << frac_log2
+    res = {}
+    t_start = time.perf_counter()
+    for prec in ("f64", "f32"):
+        cdt = np.complex128 if prec == "f64" else np.complex64
+        run = O.c2c_batched if prec == "f64" else O.c2c_batched_f32
+        buf = O.uniform_complex((m * n,), seed=12345).astype(cdt)
+
+        def passes(sign):
+            axes = [lambda: run(buf, n, m, 1, n, sign, threads),                # axis 2: rows
+                    lambda: [run(buf[i * n * n:(i + 1) * n * n], n, n, n, 1, sign, threads)
+                             for i in range(lead)],                                # axis 1
+                    lambda: run(buf, n, m, m, 1, sign, threads)]                  # axis 0
+            for f in (axes if sign < 0 else axes[::-1]):
+                f()
+        passes(-1)
+        passes(+1)                    # warm-up forward+backward
+        tf, tfb = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            passes(-1)
+            t1 = time.perf_counter()
+            passes(+1)
+            t2 = time.perf_counter()
+            tf.append((t1 - t0) * scale)
+            tfb.append((t2 - t0) * scale)
+
+        def summary(ts):
+            return {"median_ms": round(statistics.median(ts) * 1e3, 2),
+                    "mean_ms": round(statistics.mean(ts) * 1e3, 2),
+                    "sd_ms": round(statistics.stdev(ts) * 1e3, 2)}
+        res[prec] = {"forward": summary(tf), "forward_backward": summary(tfb),
+                     "forward_gflops_median": round(flops_full / statistics.median(tf) / 1e9, 3)}
+        del buf
+    return {"value": res["f64"]["forward_gflops_median"], "unit": "GFLOP/s", "cores": threads,
             "kind": "port",
-            "sample": (f"FP64 oracle (restated FFTWBackend<double> batched plans, OpenMP over "
-                       f"the batch): {m} of each axis' {n * n} length-{n} transforms, with "
-                       f"strides 1, {n} and {m} elements; {dt:.2f} s")}
+            "sample": (f"oracle's restated FFTWBackend<double> and <float> batched plans (OpenMP "
+                       f"over the batch, {threads} threads): {m} of each axis' {n * n} length-{n} "
+                       f"transforms with strides 1, {n} and {m} elements; one warm-up "
+                       f"forward+backward, then {reps} timed pairs; times scaled x{scale} to one "
+                       f"{n}^3 transform; {time.perf_counter() - t_start:.1f} s in all"),
+            "fp64": res["f64"], "fp32": res["f32"],
+            "fftw_libraries_found": fftw,
+            "note": ("FFTW is absent on this machine (find_library found neither library), so "
+                     "the column is the oracle's own plans, not FFTW" if not any(fftw.values())
+                     else "FFTW libraries were found; the column still times the oracle's plans")}
 
 
 def bench_pencil(args, torch, A):
@@ -131,7 +173,9 @@ def bench_pencil(args, torch, A):
     total = sum(alg.values())
     how = ("fused: stages store into / load from 4+4 device segments per transition"
            if share.fused else "pack/unpack into 4+4 device segments per transition")
-    print(json.dumps({
+    del buf
+    torch.cuda.empty_cache()
+    return ({
         "workload": "R2C pencil share: rank (0,0) of 2048^3 on a 4x4 grid, 512x512x2048 real, "
                     + how, "fused": share.fused,
         "precision": args.precision, "planning": args.plan, "stage_kernels": kernels,
@@ -139,8 +183,7 @@ def bench_pencil(args, torch, A):
         "part_ms": {k: round(v, 4) for k, v in parts.items()},
         "part_tbs": {k: round(alg[k] / (parts[k] * 1e-3) / 1e12, 3) for k in parts},
         "algorithmic_bytes": total, "tbs_algorithmic": round(total / (ms * 1e-3) / 1e12, 3),
-        "gflops_3d": round(2.5 * 2048 ** 3 / 16 * math.log2(2048 ** 3) / (ms * 1e-3) / 1e9, 1)}),
-        flush=True)
+        "gflops_3d": round(2.5 * 2048 ** 3 / 16 * math.log2(2048 ** 3) / (ms * 1e-3) / 1e9, 1)})
 
 
 def bench_batch(args, torch, A):
@@ -261,7 +304,10 @@ def bench_batch(args, torch, A):
                 worst, worst_item = e, f"{k}{list(s)}"
         acc = {"max_rel_l2_vs_fp64": worst, "worst_transform": worst_item}
         del x64, b64
-    print(json.dumps({
+    del graph
+    del x, x0, b
+    torch.cuda.empty_cache()
+    return ({
         "workload": "512 uneven transforms, complex<float>, seeded shapes", "transforms": len(items),
         "complex_points": points, "resident_gb": round(pos * 8 / 1e9, 3),
         "launches_per_forward": {"one_per_transform_stage": each, "grouped": grouped},
@@ -278,7 +324,84 @@ def bench_batch(args, torch, A):
             round(res["reference_driver_per_shape_graph"]["ms"] / res["grouped_graph"]["ms"], 3)
             if "reference_driver_per_shape_graph" in res else None),
         "planning": args.plan,
-        "accuracy": acc}), flush=True)
+        "accuracy": acc})
+
+
+def bench_512(args, torch, A):
+    """The 512^3 in-place C2C of the GPU plain-method case, complex<double> and
+    complex<float>: iid Uniform[-1,1) re/im plus 8 plane waves
+    exp(+2 pi i k_m . x / 512) with seeded integer wavevectors; Measure
+    planning; K forwards timed.  Accuracy without the oracle (which only the
+    tests load): the FP64 path's scaled roundtrip and Parseval's identity,
+    and the FP32 path against the FP64 path on the same input (the tests pin
+    the FP64 path to the oracle at this size)."""
+    import ctypes as C
+    n, N = 512, 512 ** 3
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(512)
+    waves = torch.randint(0, n, (8, 3), generator=g, device=dev)
+    x = torch.view_as_complex(torch.rand((N, 2), generator=g, device=dev,
+                                         dtype=torch.float64) * 2 - 1).reshape(n, n, n)
+    idx = torch.arange(n, device=dev, dtype=torch.float64)
+    for k in waves.tolist():
+        ph = (k[0] * idx[:, None, None] + k[1] * idx[None, :, None] + k[2] * idx[None, None, :])
+        x += torch.exp(1j * (2 * math.pi / n) * ph)
+        del ph
+    L = A.load()
+    stream = torch.cuda.current_stream()
+    sp = stream.cuda_stream
+    shape = (C.c_int * 3)(n, n, n)
+    flag = {"estimate": A.PFX_PLAN_ESTIMATE, "measure": A.PFX_PLAN_MEASURE,
+            "patient": A.PFX_PLAN_PATIENT}[args.plan]
+    out, y64 = {}, None
+    for prec in ("f64", "f32"):
+        code = A.PFX_F64 if prec == "f64" else A.PFX_F32
+        buf = x.clone() if prec == "f64" else x.to(torch.complex64)
+        nd = C.c_void_p()
+        A.check(L.pfx_ndplan_create(C.byref(nd), code, A.PFX_C2C, 3, shape), "pfx_ndplan_create")
+        if flag != A.PFX_PLAN_ESTIMATE:
+            scratch = torch.empty_like(buf)
+            A.check(L.pfx_ndplan_tune(nd.value, flag, scratch.data_ptr(), sp), "pfx_ndplan_tune")
+            torch.cuda.synchronize()
+            del scratch
+        work = buf.clone()
+        for _ in range(args.warmup):
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, work.data_ptr(), 1.0, sp), "fwd")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, work.data_ptr(), 1.0, sp), "fwd")
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.steps
+        work.copy_(buf)
+        A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, work.data_ptr(), 1.0, sp), "fwd")
+        torch.cuda.synchronize()
+        r = {"ms_per_forward": round(ms, 4),
+             "tbs_algorithmic": round(2 * 3 * N * buf.element_size() / (ms * 1e-3) / 1e12, 3),
+             "gflops": round(5.0 * N * math.log2(N) / (ms * 1e-3) / 1e9, 1)}
+        if prec == "f64":
+            y64 = work.clone()
+            r["parseval_rel_err"] = float(abs(torch.linalg.vector_norm(work) ** 2 /
+                                              (N * torch.linalg.vector_norm(buf) ** 2) - 1))
+            A.check(L.pfx_ndplan_execute(nd.value, A.PFX_BACKWARD, work.data_ptr(), 1.0 / N, sp),
+                    "bwd")
+            torch.cuda.synchronize()
+            r["roundtrip_rel_l2"] = float(torch.linalg.vector_norm(work - buf) /
+                                          torch.linalg.vector_norm(buf))
+        else:
+            d = work.to(torch.complex128) - y64
+            r["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(d) / torch.linalg.vector_norm(y64))
+            r["max_abs_rel_to_peak"] = float(d.abs().max() / y64.abs().max())
+            del d
+        log(f"[bench] 512^3 {prec}: {ms:.3f} ms")
+        L.pfx_ndplan_destroy(nd.value)
+        out[prec] = r
+        del buf, work
+    del x, y64
+    torch.cuda.empty_cache()
+    return {"workload": "3D C2C forward, in place, 512^3, U[-1,1) + 8 plane waves, one GPU",
+            "results": out}
 
 
 def main():
@@ -305,6 +428,9 @@ def main():
                          "the timed region's dispatches in the trace)")
     ap.add_argument("--no-split", action="store_true",
                     help="cube workload: skip the split-precision forms timed after plain FP32")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="cube workload: skip the 512^3, pencil-share and uneven-batch cases "
+                         "reported beside the headline")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 4 GiB (the library's 2D-copy "
                          "kernel) five times, to scale rocprof FETCH/WRITE_SIZE counters")
@@ -335,9 +461,11 @@ def main():
         return
 
     if args.workload == "pencil":
-        return bench_pencil(args, torch, A)
+        print(json.dumps(bench_pencil(args, torch, A)), flush=True)
+        return
     if args.workload == "batch":
-        return bench_batch(args, torch, A)
+        print(json.dumps(bench_batch(args, torch, A)), flush=True)
+        return
 
     n = args.n
     D = 3
@@ -539,7 +667,10 @@ def main():
         try:
             with open(tpath) as f:
                 t = json.load(f)
-            traffic = t.get("stages", {}).get(str(dom), {}).get("hbm_bytes_per_launch")
+            st = t.get("stages", {}).get(str(dom), {})
+            # counters of exactly the kernel this run times; otherwise none
+            if st.get("plan_kernel") == names[dom]:
+                traffic = st.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
 
@@ -628,6 +759,26 @@ def main():
             log(f"[bench] {mode}: {sms:.3f} ms/step ({ms_per_step / sms:.3f}x plain)")
             L.pfx_ndplan_destroy(nds.value)
         out["split_vs_plain"] = split
+    # the other measured cases on the same run (the driver records only this
+    # line): 512^3 FP64/FP32, one rank's R2C pencil share of 2048^3 on 4x4,
+    # and the 512 uneven transforms
+    if n == 1024 and prec == A.PFX_F32 and not args.no_extra:
+        import copy
+        del x
+        torch.cuda.empty_cache()
+        extra = {}
+        xargs = copy.copy(args)
+        xargs.no_reference_driver = True   # its 512 graphs alone take longer than the rest
+        for label, fn in (("512cube", bench_512), ("pencil_share", bench_pencil),
+                          ("uneven_batch", bench_batch)):
+            t0 = time.perf_counter()
+            try:
+                extra[label] = fn(xargs, torch, A)
+            except Exception as e:  # reported, the headline stands
+                extra[label] = {"error": f"{type(e).__name__}: {e}"}
+            log(f"[bench] {label}: {time.perf_counter() - t0:.1f} s")
+            torch.cuda.empty_cache()
+        out["configs"] = extra
     # the metric's accuracy bound (rel-L2 <= 2e-6 against FP64) for every
     # FP32 form measured; a run that breaks it exits non-zero after printing
     ok = None

@@ -115,6 +115,33 @@ def c2c_batched(data: np.ndarray, length: int, batch: int, stride: int, dist: in
     return data
 
 
+_lib_f32 = None
+
+
+def lib_f32():
+    """The FFT core built in single precision (oracle/_build/liborc_f32.so):
+    the FFTWBackend<float> counterpart timed by the benchmark's CPU baseline."""
+    global _lib_f32
+    if _lib_f32 is None:
+        path = os.path.join(_HERE, "_build", "liborc_f32.so")
+        if not os.path.exists(path):
+            build(force=True)
+        L = C.CDLL(path)
+        L.orc_c2c_batched.argtypes = [C.POINTER(C.c_float), _i64, _i64, _i64, _i64, C.c_int,
+                                      C.c_int]
+        _lib_f32 = L
+    return _lib_f32
+
+
+def c2c_batched_f32(data: np.ndarray, length: int, batch: int, stride: int, dist: int,
+                    sign: int = -1, nthreads: int = 0) -> np.ndarray:
+    """orc_c2c_batched computed in float on complex64 data, in place."""
+    assert data.dtype == np.complex64 and data.flags.c_contiguous
+    lib_f32().orc_c2c_batched(data.view(np.float32).ctypes.data_as(C.POINTER(C.c_float)),
+                              length, batch, stride, dist, sign, nthreads)
+    return data
+
+
 def fft(x: np.ndarray, sign: int = -1) -> np.ndarray:
     """1D DFT of a contiguous vector (out of place)."""
     y = np.ascontiguousarray(x, dtype=np.complex128).copy()

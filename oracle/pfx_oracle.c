@@ -39,7 +39,13 @@
 #include <omp.h>
 #endif
 
-typedef struct { double re, im; } cplx;
+/* The scalar of the FFT core: double for the oracle; the CPU baseline of the
+ * benchmark also builds the core alone with ORC_REAL=float (ORC_FFT_ONLY),
+ * the FFTWBackend<float> counterpart. */
+#ifndef ORC_REAL
+#define ORC_REAL double
+#endif
+typedef struct { ORC_REAL re, im; } cplx;
 
 /* ------------------------------------------------------------------------ */
 /* Twiddle table: w[k] = exp(sign * 2 pi i k / n), long double then rounded. */
@@ -50,8 +56,8 @@ static cplx *make_twiddles(int64_t n, int sign) {
   for (int64_t k = 0; k < n; ++k) {
     /* reduce k/n to the first octant-free form: angle = 2 pi k / n */
     long double a = 2.0L * pi * (long double)k / (long double)n;
-    w[k].re = (double)cosl(a);
-    w[k].im = (double)(sign * sinl(a));
+    w[k].re = (ORC_REAL)cosl(a);
+    w[k].im = (ORC_REAL)(sign * sinl(a));
   }
   return w;
 }
@@ -91,7 +97,7 @@ static void fft_rec(const cplx *in, int64_t is, cplx *out, int64_t n,
       tmp[r].im = y.re * w.im + y.im * w.re;
     }
     for (int64_t q = 0; q < p; ++q) {
-      double sr = 0.0, si = 0.0;
+      ORC_REAL sr = 0, si = 0;
       for (int64_t r = 0; r < p; ++r) {
         cplx w = tw[((r * q) % p) * pstep];
         sr += tmp[r].re * w.re - tmp[r].im * w.im;
@@ -118,7 +124,7 @@ static int64_t largest_prime_factor(int64_t n) {
  * FFTWBackend<double>::create_stage_plan + execute_stage
  * (fft_backend_fftw.hpp:387-424, 562-572).  nthreads <= 0: all threads.
  */
-int orc_c2c_batched(double *data, int64_t length, int64_t batch, int64_t stride,
+int orc_c2c_batched(ORC_REAL *data, int64_t length, int64_t batch, int64_t stride,
                     int64_t dist, int sign, int nthreads) {
   if (length <= 0 || batch < 0) return -1;
   if (batch == 0) return 0;
@@ -148,6 +154,7 @@ int orc_c2c_batched(double *data, int64_t length, int64_t batch, int64_t stride,
   return 0;
 }
 
+#ifndef ORC_FFT_ONLY
 /* Brute-force O(n^2) DFT in long double; out-of-place, contiguous. */
 int orc_dft_brute(const double *in, double *out, int64_t n, int sign) {
   const long double pi = 3.141592653589793238462643383279502884L;
@@ -582,3 +589,4 @@ int orc_max_threads(void) {
   return 1;
 #endif
 }
+#endif /* ORC_FFT_ONLY */

@@ -155,3 +155,24 @@ def test_planted_mistakes_are_caught():
     assert not _closed_form_checks(drop)
     # normalised instead of FFTW's unnormalised convention
     assert not _closed_form_checks(lambda x: O.fftn(x) / x.size)
+
+
+@pytest.mark.parametrize("n,batch,stride,dist", [(1024, 8, 1, 1024), (48, 20, 20, 1),
+                                                  (120, 6, 1, 130), (2048, 3, 3, 1)])
+def test_float_core_matches_double_oracle(n, batch, stride, dist):
+    """The single-precision build of the oracle's FFT core (the benchmark's
+    FP32 CPU baseline) agrees with the FP64 oracle to FP32 accuracy on the
+    same inputs, both directions, in every layout."""
+    size = (batch - 1) * dist + (n - 1) * stride + 1
+    x = O.uniform_complex((size,), seed=n + batch).astype(np.complex64)
+    for sign in (-1, +1):
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, stride, dist, sign)
+        got = x.copy()
+        O.c2c_batched_f32(got, n, batch, stride, dist, sign)
+        idx = (np.arange(batch)[:, None] * dist + np.arange(n)[None, :] * stride).ravel()
+        assert O.rel_l2(got[idx], ref[idx]) <= 1e-6 * np.log2(n)
+        # elements outside the transforms are untouched
+        mask = np.ones(size, bool)
+        mask[idx] = False
+        assert np.array_equal(got[mask], x[mask])
