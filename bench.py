@@ -428,6 +428,12 @@ def main():
                          "the timed region's dispatches in the trace)")
     ap.add_argument("--no-split", action="store_true",
                     help="cube workload: skip the split-precision forms timed after plain FP32")
+    ap.add_argument("--tune-cache-save", default=None,
+                    help="write the tuning cache (every plan's chosen kernel) to this file at "
+                         "the end, so a profiling run can replay the same kernels")
+    ap.add_argument("--tune-cache-load", default=None,
+                    help="load a tuning cache before planning (Measure then takes the cached "
+                         "kernels without timing)")
     ap.add_argument("--no-extra", action="store_true",
                     help="cube workload: skip the 512^3, pencil-share and uneven-batch cases "
                          "reported beside the headline")
@@ -447,6 +453,8 @@ def main():
     from parafaft_amd import _capi as A
 
     L = A.load()
+    if args.tune_cache_load:
+        A.check(L.pfx_tune_cache_load(args.tune_cache_load.encode()), "pfx_tune_cache_load")
     if args.calibrate_copy:
         nbytes = 4 << 30
         a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
@@ -460,11 +468,11 @@ def main():
         print(json.dumps({"calibrate_copy_bytes": nbytes, "launches": 5}), flush=True)
         return
 
-    if args.workload == "pencil":
-        print(json.dumps(bench_pencil(args, torch, A)), flush=True)
-        return
-    if args.workload == "batch":
-        print(json.dumps(bench_batch(args, torch, A)), flush=True)
+    if args.workload in ("pencil", "batch"):
+        r = (bench_pencil if args.workload == "pencil" else bench_batch)(args, torch, A)
+        if args.tune_cache_save:
+            A.check(L.pfx_tune_cache_save(args.tune_cache_save.encode()), "pfx_tune_cache_save")
+        print(json.dumps(r), flush=True)
         return
 
     n = args.n
@@ -787,6 +795,8 @@ def main():
                          if v.get("accuracy")]
         ok = all(f["rel_l2_vs_fp64"] <= ACC_BOUND for f in forms)
     out["accuracy_ok"] = ok
+    if args.tune_cache_save:
+        A.check(L.pfx_tune_cache_save(args.tune_cache_save.encode()), "pfx_tune_cache_save")
     print(json.dumps(out), flush=True)
     if ok is False:
         log(f"[bench] accuracy bound rel-L2 <= {ACC_BOUND} vs FP64 exceeded")

@@ -201,13 +201,17 @@ bool cached_choice(const pfx_plan_t *p, const KernelEntry *k0, const KernelEntry
   return kernel_fits_plan(p, k0, *e, *order);
 }
 
-// Valid-looking plan key: the fields plan_key writes, in its order.
+// Valid-looking plan key: the fields plan_key writes, in its order, and
+// optionally the chunk suffix of an in-context choice (" chunkP ofN").
 bool is_plan_key(const char *s) {
-  int pr, kd;
+  int pr, kd, P, n0;
   long long n, i, o, st, d, dout;
   char tail;
-  return sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld%c", &pr, &kd, &n, &i, &o, &st,
-                &d, &dout, &tail) == 8;
+  int fields = sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld%c", &pr, &kd, &n, &i, &o,
+                      &st, &d, &dout, &tail);
+  if (fields == 8) return true;
+  return sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld chunk%d of%d%c", &pr, &kd, &n,
+                &i, &o, &st, &d, &dout, &P, &n0, &tail) == 10;
 }
 
 }  // namespace
@@ -1256,6 +1260,22 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
     for (int st = 0; st + 1 < p->ndim; ++st) {
       pfx_plan part = p->cstages[st];
       if (!part->kernel) continue;
+      // the in-context choice is cached under the chunk plan's key with a
+      // suffix (a cache file replays it, as for single plans)
+      const std::string ctx_key = plan_key(part) + " chunk" + std::to_string(p->chunk) +
+                                  " of" + std::to_string(p->shape[0]);
+      {
+        std::lock_guard<std::mutex> lock(g_tune_mu);
+        auto it = g_tune.find(ctx_key);
+        const KernelEntry *e = nullptr;
+        int order = 1;
+        if (it != g_tune.end() && parse_tuned(it->second, &e, &order) &&
+            kernel_fits_plan(part, part->kernel, e, order)) {
+          PFX_HIP(set_kernel(part, e), "chunk kernel");
+          part->xcd_remap = order;
+          continue;
+        }
+      }
       const KernelEntry *best = part->kernel;
       int best_order = part->xcd_remap;
       float best_ms = -1.f;
@@ -1276,6 +1296,8 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
       }
       PFX_HIP(set_kernel(part, best), "chunk kernel");
       part->xcd_remap = best_order;
+      std::lock_guard<std::mutex> lock(g_tune_mu);
+      g_tune[ctx_key] = entry_id(best) + "|order=" + std::to_string(best_order);
     }
     hipEventDestroy(a);
     hipEventDestroy(b);
