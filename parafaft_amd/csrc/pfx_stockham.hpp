@@ -265,7 +265,12 @@ template <int N> struct SegTabs {
 // NT: non-temporal HBM accesses (streaming, no cache allocation) for a
 // pass whose data is not read again soon; measured 4% faster on the
 // 1024^3 axis-0 column pattern when both loads and stores stream.
+template <bool B> struct BoolC {
+  static constexpr bool value = B;
+};
+
 template <typename T, bool SEG = false, bool TAB = false, bool NT = false> struct GlobalPtr {
+  static constexpr bool kSeg = false;
   V2<T>* p;
   const V2<T>* pin;          // load side (== p in place)
   int64_t stride;
@@ -303,9 +308,13 @@ template <typename T, bool SEG = false, bool TAB = false, bool NT = false> struc
       *q = x;
     }
   }
+  // the segmented form's accessors with the map branch already taken
+  template <bool M> __device__ __forceinline__ V2<T> load_as(int e) const { return load(e); }
+  template <bool M> __device__ __forceinline__ void store_as(int e, V2<T> v) const { store(e, v); }
 };
 
 template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
+  static constexpr bool kSeg = true;
   V2<T>* p;
   const V2<T>* pin;
   int64_t stride;
@@ -316,7 +325,7 @@ template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
   KSegMap* in_map;           // non-null: loads come from a segment layout
   KSegMap* out_map;          // non-null: stores go to a segment layout
   uint32_t in_lead, out_lead;  // transform's place in each map
-  int64_t in_trail, out_trail;
+  uint32_t in_trail, out_trail;  // (< trailing < 2^32)
   int lane_j;                // this lane's position among its transform's threads
   int blk;                   // threads per transform
   // TAB: the maps' LDS tables (set by the kernel body after building them)
@@ -324,7 +333,7 @@ template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
   const uint16_t* out_ent;
   const SegPart* in_parts;
   const SegPart* out_parts;
-  int64_t in_trailing, out_trailing;
+  uint32_t in_trailing, out_trailing;
   // first element of the wave's block (e - lane_j is the same on every lane)
   __device__ __forceinline__ int block_start(int e) const {
     return __builtin_amdgcn_readfirstlane(e - lane_j);
@@ -333,27 +342,30 @@ template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
   // flat) two-element vectors, one load or store instruction per element
   typedef T E2 __attribute__((ext_vector_type(2)));
   using GV = __attribute__((address_space(1))) E2;
+  // (trailing < 2^32 and trail < trailing: one 32 x 32 + 64-bit multiply-add)
   __device__ __forceinline__ static GV* tab_addr(const uint16_t* ent, const SegPart* parts,
-                                                 uint32_t lead, int64_t trail,
-                                                 int64_t trailing, int e) {
+                                                 uint32_t lead, uint32_t trail,
+                                                 uint32_t trailing, int e) {
     const uint32_t t = ent[e];
     const SegPart sp = parts[t >> 12];
     const uint32_t row = lead * (uint32_t)sp.n + (t & 0xFFFu);
-    return (GV*)sp.seg + ((uint64_t)row * (uint64_t)trailing + trail);
+    return (GV*)sp.seg + ((uint64_t)row * trailing + trail);
   }
-  __device__ __forceinline__ V2<T> load(int e) const {
-    if (in_map) {
+  // M: the map's side of the branch (true: the segment layout)
+  template <bool M> __device__ __forceinline__ V2<T> load_as(int e) const {
+    if constexpr (M) {
       if constexpr (TAB) {
         const E2 x = *tab_addr(in_ent, in_parts, in_lead, in_trail, in_trailing, e);
         return mk<T>(x.x, x.y);
       } else {
         return *seg_addr<T>(in_map, in_lead, in_trail, e, block_start(e), blk);
       }
+    } else {
+      return pin[(int64_t)e * stride];
     }
-    return pin[(int64_t)e * stride];
   }
-  __device__ __forceinline__ void store(int e, V2<T> v) const {
-    if (out_map) {
+  template <bool M> __device__ __forceinline__ void store_as(int e, V2<T> v) const {
+    if constexpr (M) {
       if constexpr (TAB) {
         E2 x;
         x.x = v.x;
@@ -366,7 +378,34 @@ template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
       p[(int64_t)e * stride] = v;
     }
   }
+  __device__ __forceinline__ V2<T> load(int e) const {
+    return in_map ? load_as<true>(e) : load_as<false>(e);
+  }
+  __device__ __forceinline__ void store(int e, V2<T> v) const {
+    if (out_map) store_as<true>(e, v);
+    else store_as<false>(e, v);
+  }
 };
+
+// f(BoolC<M>) with M the side of g's load (store) map branch, taken once for
+// a whole loop of accesses rather than per access: the plain and segmented
+// loops are then straight-line code each.
+template <class G, class F> __device__ __forceinline__ void with_in_map(const G& g, F&& f) {
+  if constexpr (G::kSeg) {
+    if (g.in_map) f(BoolC<true>());
+    else f(BoolC<false>());
+  } else {
+    f(BoolC<false>());
+  }
+}
+template <class G, class F> __device__ __forceinline__ void with_out_map(const G& g, F&& f) {
+  if constexpr (G::kSeg) {
+    if (g.out_map) f(BoolC<true>());
+    else f(BoolC<false>());
+  } else {
+    f(BoolC<false>());
+  }
+}
 
 // Build the LDS tables of the maps a segmented launch uses and point g at
 // them.  Every thread of the workgroup calls this (it ends in a barrier).
@@ -380,8 +419,8 @@ __device__ __forceinline__ void seg_tables(GlobalPtr<T, true, true, false>& g, u
   g.out_ent = ST::ent(mem, 1);
   g.in_parts = ST::parts(mem, 0);
   g.out_parts = ST::parts(mem, 1);
-  g.in_trailing = g.in_map ? g.in_map->trailing : 0;
-  g.out_trailing = g.out_map ? g.out_map->trailing : 0;
+  g.in_trailing = g.in_map ? (uint32_t)g.in_map->trailing : 0u;
+  g.out_trailing = g.out_map ? (uint32_t)g.out_map->trailing : 0u;
 }
 
 // LDS budget of one workgroup; a segmented kernel builds the tables when its
@@ -909,17 +948,37 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
       bfly<T, R>(v + q * R);
     }
     if constexpr (sizeof...(Rest) == 0) {
-      // last pass: outputs straight to HBM
+      // last pass: outputs straight to HBM (segmented: the map branch is
+      // taken once for the whole loop, which keeps the plain and segment
+      // store loops straight-line code; the plain kernels keep their own)
+      auto stores = [&](auto M) __attribute__((always_inline)) {
 #pragma unroll
-      for (int q = 0; q < Q; ++q) {
-        const int jj = j + q * TPT;
-        const int ob = (jj / Ns) * Ns * R + jj % Ns;
+        for (int q = 0; q < Q; ++q) {
+          const int jj = j + q * TPT;
+          const int ob = (jj / Ns) * Ns * R + jj % Ns;
 #pragma unroll
-        for (int m = 0; m < R; ++m) {
-          V2<T> x = v[q * R + m];
-          if (g.use_scale) x = scl<T>(x, g.scale);
-          if (g.swap_io) x = swp<T>(x);
-          if (g.valid) g.store(ob + m * Ns, x);
+          for (int m = 0; m < R; ++m) {
+            V2<T> x = v[q * R + m];
+            if (g.use_scale) x = scl<T>(x, g.scale);
+            if (g.swap_io) x = swp<T>(x);
+            if (g.valid) g.template store_as<decltype(M)::value>(ob + m * Ns, x);
+          }
+        }
+      };
+      if constexpr (G::kSeg) {
+        with_out_map(g, stores);
+      } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+          const int jj = j + q * TPT;
+          const int ob = (jj / Ns) * Ns * R + jj % Ns;
+#pragma unroll
+          for (int m = 0; m < R; ++m) {
+            V2<T> x = v[q * R + m];
+            if (g.use_scale) x = scl<T>(x, g.scale);
+            if (g.swap_io) x = swp<T>(x);
+            if (g.valid) g.store(ob + m * Ns, x);
+          }
         }
       }
     } else {
@@ -1016,14 +1075,30 @@ __device__ __forceinline__ void c2c_wide_body_in(const StockhamArgs& a, int64_t 
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
   V2<T> v[E];
+  if constexpr (SEG) {
+    // the map branch taken once for the whole first-pass load loop
+    with_in_map(g, [&](auto M) __attribute__((always_inline)) {
 #pragma unroll
-  for (int q = 0; q < Q0; ++q) {
-    const int jj = j + q * TPT;
+      for (int q = 0; q < Q0; ++q) {
+        const int jj = j + q * TPT;
 #pragma unroll
-    for (int m = 0; m < R0; ++m) {
-      V2<T> x = mk<T>(T(0), T(0));
-      if (g.valid) x = g.load(jj + m * NB0);
-      v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
+        for (int m = 0; m < R0; ++m) {
+          V2<T> x = mk<T>(T(0), T(0));
+          if (g.valid) x = g.template load_as<decltype(M)::value>(jj + m * NB0);
+          v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
+        }
+      }
+    });
+  } else {
+#pragma unroll
+    for (int q = 0; q < Q0; ++q) {
+      const int jj = j + q * TPT;
+#pragma unroll
+      for (int m = 0; m < R0; ++m) {
+        V2<T> x = mk<T>(T(0), T(0));
+        if (g.valid) x = g.load(jj + m * NB0);
+        v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
+      }
     }
   }
   if constexpr (!GTW && EARLY) tstage.commit(B::tw(smem));
