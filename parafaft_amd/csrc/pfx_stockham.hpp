@@ -262,6 +262,46 @@ template <int N> struct SegTabs {
   }
 };
 
+// Part tables of the cursor form, one per map side: entry q holds segment
+// q's address, block count and first element; entry nparts is a sentinel
+// whose first element exceeds every index.  A lane whose accesses along the
+// split axis come in increasing order keeps a cursor on its current part
+// and moves it forward past a boundary (rare: blocks are long), so an
+// access is one 32-bit add and one multiply-add, with no table read.
+struct SegPartS {
+  void* seg;
+  int n;
+  int s;
+};
+static_assert(sizeof(SegPartS) == 16, "part table entries are 16 bytes");
+struct SegCursorTabs {
+  static constexpr int side_bytes = (kMaxSegPeers + 1) * (int)sizeof(SegPartS);
+  static constexpr int bytes = 2 * side_bytes;
+  __device__ static __forceinline__ SegPartS* parts(unsigned char* m, int side) {
+    return reinterpret_cast<SegPartS*>(m + side * side_bytes);
+  }
+  // all threads of the workgroup take part; a barrier must follow
+  __device__ static __forceinline__ void fill(unsigned char* m, int side, KSegMap* map) {
+    SegPartS* pp = parts(m, side);
+    const int np = map->nparts;
+    for (int q = 0; q < np; ++q)
+      if ((int)threadIdx.x == q) {
+        SegPartS sp;
+        sp.seg = map->seg[q];
+        sp.n = map->n[q];
+        sp.s = map->s[q];
+        pp[q] = sp;
+      }
+    if ((int)threadIdx.x == np) {
+      SegPartS sp;
+      sp.seg = nullptr;
+      sp.n = 0;
+      sp.s = 0x7fffffff;
+      pp[np] = sp;
+    }
+  }
+};
+
 // NT: non-temporal HBM accesses (streaming, no cache allocation) for a
 // pass whose data is not read again soon; measured 4% faster on the
 // 1024^3 axis-0 column pattern when both loads and stores stream.
@@ -378,6 +418,56 @@ template <typename T, bool TAB, bool NT> struct GlobalPtr<T, true, TAB, NT> {
       p[(int64_t)e * stride] = v;
     }
   }
+  // cursor form (the kernel body sets the tables after filling them)
+  const SegPartS* in_pp;
+  const SegPartS* out_pp;
+  struct Cursor {
+    GV* base;    // the current part's segment
+    int rowoff;  // lead * n - s of the current part: row of element e is rowoff + e
+    int snext;   // first element of the next part
+    int p;
+  };
+  __device__ __forceinline__ static void cur_set(Cursor& c, const SegPartS* pp, uint32_t lead) {
+    const SegPartS sp = pp[c.p];
+    c.base = (GV*)sp.seg;
+    c.rowoff = (int)(lead * (uint32_t)sp.n) - sp.s;
+    c.snext = pp[c.p + 1].s;
+  }
+  // a cursor on the part holding e0, the smallest element the lane accesses
+  __device__ __forceinline__ static Cursor cur_begin(const SegPartS* pp, uint32_t lead, int e0) {
+    Cursor c;
+    c.p = 0;
+    while (e0 >= pp[c.p + 1].s) ++c.p;
+    cur_set(c, pp, lead);
+    return c;
+  }
+  // address of element e >= every element this cursor has addressed before
+  __device__ __forceinline__ static GV* cur_addr(Cursor& c, const SegPartS* pp, uint32_t lead,
+                                                 uint32_t trail, uint32_t trailing, int e) {
+    if (e >= c.snext) {
+      do {
+        ++c.p;
+      } while (e >= pp[c.p + 1].s);
+      cur_set(c, pp, lead);
+    }
+    return c.base + ((uint64_t)(uint32_t)(c.rowoff + e) * trailing + trail);
+  }
+  __device__ __forceinline__ Cursor in_cursor(int e0) const {
+    return cur_begin(in_pp, in_lead, e0);
+  }
+  __device__ __forceinline__ Cursor out_cursor(int e0) const {
+    return cur_begin(out_pp, out_lead, e0);
+  }
+  __device__ __forceinline__ V2<T> load_cur(Cursor& c, int e) const {
+    const E2 x = *cur_addr(c, in_pp, in_lead, in_trail, in_trailing, e);
+    return mk<T>(x.x, x.y);
+  }
+  __device__ __forceinline__ void store_cur(Cursor& c, int e, V2<T> v) const {
+    E2 x;
+    x.x = v.x;
+    x.y = v.y;
+    *cur_addr(c, out_pp, out_lead, out_trail, out_trailing, e) = x;
+  }
   __device__ __forceinline__ V2<T> load(int e) const {
     return in_map ? load_as<true>(e) : load_as<false>(e);
   }
@@ -423,12 +513,29 @@ __device__ __forceinline__ void seg_tables(GlobalPtr<T, true, true, false>& g, u
   g.out_trailing = g.out_map ? (uint32_t)g.out_map->trailing : 0u;
 }
 
+// The cursor form's part tables (every thread calls this; it ends in a
+// barrier).
+template <typename T, bool TAB, bool NT>
+__device__ __forceinline__ void seg_cursor_tables(GlobalPtr<T, true, TAB, NT>& g,
+                                                  unsigned char* mem) {
+  if (g.in_map) SegCursorTabs::fill(mem, 0, g.in_map);
+  if (g.out_map) SegCursorTabs::fill(mem, 1, g.out_map);
+  __syncthreads();
+  g.in_pp = SegCursorTabs::parts(mem, 0);
+  g.out_pp = SegCursorTabs::parts(mem, 1);
+  g.in_trailing = g.in_map ? (uint32_t)g.in_map->trailing : 0u;
+  g.out_trailing = g.out_map ? (uint32_t)g.out_map->trailing : 0u;
+}
+
 // LDS budget of one workgroup; a segmented kernel builds the tables when its
 // own block leaves room for them.
 constexpr int kLdsMax = 163840;
 template <bool SEG, int N, int BYTES> constexpr bool seg_tab_fits() {
   if constexpr (!SEG || N > 4096) return false;
   else return BYTES + SegTabs<N>::bytes <= kLdsMax;
+}
+template <bool SEG, int BYTES> constexpr bool seg_cursor_fits() {
+  return SEG && BYTES + SegCursorTabs::bytes <= kLdsMax;
 }
 
 __device__ __forceinline__ int64_t xform_offset(int64_t b, const StockhamArgs& a) {
@@ -920,14 +1027,15 @@ struct WidePasses;
 
 template <typename T, int N, int TPT, class WL, int E, int PI, int Ns>
 struct WidePasses<T, N, TPT, WL, E, PI, Ns> {
-  template <class G>
+  template <bool CUR = false, class G>
   __device__ static __forceinline__ void run(int, int, V2<T>*, T*, const V2<T>*, const G&) {}
 };
 
 template <typename T, int N, int TPT, class WL, int E, int PI, int Ns, int R, int... Rest>
 struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
   static_assert(E % R == 0, "radix must divide the per-thread element count");
-  template <class G>
+  // CUR: the segmented stores of the last pass go through a cursor
+  template <bool CUR = false, class G>
   __device__ static __forceinline__ void run(int j, int c, V2<T>* v, T* lds, const V2<T>* tw, const G& g) {
     constexpr int Q = E / R;
     // twiddle + butterflies on the values this thread holds
@@ -965,7 +1073,28 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
           }
         }
       };
-      if constexpr (G::kSeg) {
+      if constexpr (CUR) {
+        // last pass: Ns R = N, so element j + TPT (q + Q m) in (m, q) order
+        // is increasing and one cursor walks the segments
+        static_assert(Ns * R == N, "the last pass spans the transform");
+        with_out_map(g, [&](auto M) __attribute__((always_inline)) {
+          if constexpr (decltype(M)::value) {
+            auto cur = g.out_cursor(j);
+#pragma unroll
+            for (int m = 0; m < R; ++m) {
+#pragma unroll
+              for (int q = 0; q < Q; ++q) {
+                V2<T> x = v[q * R + m];
+                if (g.use_scale) x = scl<T>(x, g.scale);
+                if (g.swap_io) x = swp<T>(x);
+                if (g.valid) g.store_cur(cur, j + q * TPT + m * Ns, x);
+              }
+            }
+          } else {
+            stores(BoolC<false>());
+          }
+        });
+      } else if constexpr (G::kSeg) {
         with_out_map(g, stores);
       } else {
 #pragma unroll
@@ -1011,7 +1140,7 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
         }
         __syncthreads();
       }
-      WidePasses<T, N, TPT, WL, E, PI + 1, Ns * R, Rest...>::run(j, c, v, lds, tw, g);
+      WidePasses<T, N, TPT, WL, E, PI + 1, Ns * R, Rest...>::template run<CUR>(j, c, v, lds, tw, g);
     }
   }
 };
@@ -1020,8 +1149,10 @@ struct WidePasses<T, N, TPT, WL, E, PI, Ns, R, Rest...> {
 template <typename T, int TPT, int C, bool ROWL, bool SEG, bool GTW, int... Rs> struct WideLds {
   static constexpr int N = (Rs * ... * 1);
   using B = LdsBlock<T, T, (!GTW && sizeof...(Rs) > 1 ? N : 0), WideLayout<N, C, ROWL, GTW>::size>;
-  static constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
-  static constexpr int bytes = B::bytes + (TAB ? SegTabs<(TAB ? N : 1)>::bytes : 0);
+  // segmented: cursors on a part table where it fits, else the kernel
+  // arguments' map searched per wave
+  static constexpr bool CUR = seg_cursor_fits<SEG, B::bytes>();
+  static constexpr int bytes = B::bytes + (CUR ? SegCursorTabs::bytes : 0);
 };
 
 // GTW: twiddles read from the global table (always for the row layout; a
@@ -1037,7 +1168,7 @@ __device__ __forceinline__ void c2c_wide_body_in(const StockhamArgs& a, int64_t 
   constexpr int NB0 = N / R0;
   using WL = WideLayout<N, C, ROWL, GTW>;
   using B = LdsBlock<T, T, (!GTW && sizeof...(Rs) > 1 ? N : 0), WL::size>;
-  constexpr bool TAB = seg_tab_fits<SEG, N, B::bytes>();
+  constexpr bool CUR = seg_cursor_fits<SEG, B::bytes>();
   T* lds = B::data(smem);
   const V2<T>* tw;
   int c, j;
@@ -1063,19 +1194,47 @@ __device__ __forceinline__ void c2c_wide_body_in(const StockhamArgs& a, int64_t 
     tile = tile_order(tile, ntiles, a.xcd_remap);
   }
   const int64_t b = tile * C + c;
-  GlobalPtr<T, SEG, TAB, NT && !SEG> g;
+  GlobalPtr<T, SEG, false, NT && !SEG> g;
   init_global(g, a, b);
   if constexpr (SEG) {
     g.lane_j = j;
     g.blk = TPT;
   }
-  if constexpr (TAB) seg_tables<T, N>(g, smem + B::bytes);
+  if constexpr (CUR) seg_cursor_tables(g, smem + B::bytes);
   g.stride = a.stride;
   g.swap_io = a.backward != 0;
   g.scale = T(a.scale);
   g.use_scale = a.use_scale != 0;
   V2<T> v[E];
-  if constexpr (SEG) {
+  if constexpr (CUR) {
+    // Element j + TPT (q + Q0 m) in (m, q) order: increasing, so one cursor
+    // walks the segments (the map branch is taken once for the loop).
+    with_in_map(g, [&](auto M) __attribute__((always_inline)) {
+      if constexpr (decltype(M)::value) {
+        auto cur = g.in_cursor(j);
+#pragma unroll
+        for (int m = 0; m < R0; ++m) {
+#pragma unroll
+          for (int q = 0; q < Q0; ++q) {
+            V2<T> x = mk<T>(T(0), T(0));
+            if (g.valid) x = g.load_cur(cur, j + q * TPT + m * NB0);
+            v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int q = 0; q < Q0; ++q) {
+          const int jj = j + q * TPT;
+#pragma unroll
+          for (int m = 0; m < R0; ++m) {
+            V2<T> x = mk<T>(T(0), T(0));
+            if (g.valid) x = g.template load_as<false>(jj + m * NB0);
+            v[q * R0 + m] = g.swap_io ? swp<T>(x) : x;
+          }
+        }
+      }
+    });
+  } else if constexpr (SEG) {
     // the map branch taken once for the whole first-pass load loop
     with_in_map(g, [&](auto M) __attribute__((always_inline)) {
 #pragma unroll
@@ -1102,7 +1261,7 @@ __device__ __forceinline__ void c2c_wide_body_in(const StockhamArgs& a, int64_t 
     }
   }
   if constexpr (!GTW && EARLY) tstage.commit(B::tw(smem));
-  WidePasses<T, N, TPT, WL, E, 0, 1, Rs...>::run(j, c, v, lds, tw, g);
+  WidePasses<T, N, TPT, WL, E, 0, 1, Rs...>::template run<CUR>(j, c, v, lds, tw, g);
 }
 
 template <typename T, int TPT, int C, bool ROWL, bool SEG, bool NT, int... Rs>
