@@ -56,6 +56,13 @@ const char *pfx_version(void);
  * C2C plan: `batch` transforms of `length` complex elements; element j of
  * transform b is data[b*dist + j*stride] (complex units).  Both directions
  * share one plan.
+ * Lengths: one tiled kernel where the length has a radix plan (up to 4096);
+ * else the direct O(n^2) DFT up to 4096 (complex<float>) / 2048
+ * (complex<double>); beyond that a four-step plan when the length splits
+ * into two lengths with radix plans (n = n1 * n2, e.g. 8192 = 64 * 128, up
+ * to 4096^2): two tiled passes, a twiddle pass and a transpose through a
+ * scratch buffer of the plan's footprint that the plan owns (so executions
+ * of one such plan must not overlap).  Other lengths: PFX_ERR_UNSUPPORTED.
  * Replaces: HipFFTBackend::create_stage_plan -> hipfftPlanMany(Z2Z/C2C)
  *   (backend/hipfft/fft_backend_hipfft.hpp:294-314); geometry as in
  *   FFTWBackend::create_stage_plan (backend/fftw3/fft_backend_fftw.hpp:387-424).
@@ -78,6 +85,10 @@ int pfx_plan_create_c2c_many(pfx_plan *plan, int precision, int64_t length,
  * In-place padded R2C plan: row b starts at scalar b*dist_scalars; its
  * `length` reals are replaced by length/2+1 complex values (complex row
  * distance dist_scalars/2).  stride must be 1 (the only value ParaFaFT uses).
+ * Lengths: even lengths whose half has a radix plan run one kernel; others
+ * up to the direct-DFT limit the fallback; longer even lengths the
+ * half-length C2C plan (as above) of the rows read as complex pairs, with a
+ * post-pass (C2R: a pre-pass).
  * Replaces: HipFFTBackend::create_r2c_inplace_plan (fft_backend_hipfft.hpp:361-384),
  *   FFTWBackend::create_r2c_inplace_plan (fft_backend_fftw.hpp:439-469).
  */

@@ -130,17 +130,36 @@ int base_prec(int precision) {
 
 }  // namespace
 
+// A length with no single-kernel plan beyond the direct-DFT limit.
+//  C2C: four-step, n = n1 * n2 with tiled n1 and n2 (pfx_fourstep.hip has
+//  the element-wise steps): `a` transforms along j2 (length n2, element
+//  stride n1 s) from the caller's array into `scratch` at the same offsets,
+//  the twiddle W_n^(j1 k2) follows, `c` transforms along j1 (length n1) in
+//  place on the scratch, and the transpose writes n2 k1 + k2 back.
+//  R2C / C2R (even real length n = 2h): `c` is the length-h C2C plan of the
+//  padded rows read as complex pairs, with the R2C post-pass / C2R
+//  pre-pass around it.
+struct pfx_plan_t;
+struct LongPlan {
+  int64_t n1 = 0, n2 = 0;
+  pfx_plan_t *a = nullptr;
+  pfx_plan_t *c = nullptr;
+  void *scratch = nullptr;  // one outer slice's footprint of the caller's layout
+  void *tw = nullptr;       // C2C: W_n^i, i < n; real: W_n^k, k < n
+};
+
 struct pfx_plan_t {
   int precision = PFX_F32;
   int kind = PFX_C2C;
   int64_t length = 0;     // transform length (real length for R2C/C2R)
   int64_t inner = 0, outer = 1;
   int64_t stride = 1, dist = 0, dist_outer = 0;
-  const KernelEntry *kernel = nullptr;  // null: direct-DFT fallback
+  const KernelEntry *kernel = nullptr;  // null: direct-DFT fallback or `lp`
   int xcd_remap = 1;
   void *tw = nullptr;
   void *ptw = nullptr;  // per-pass table of the tiled kernel
   int64_t tw_len = 0;
+  LongPlan *lp = nullptr;  // long lengths (kernel is null)
 };
 
 namespace {
@@ -249,6 +268,50 @@ const char *pfx_last_error_message(void) { return g_last_error.c_str(); }
 const char *pfx_version(void) { return "parafaft-mi355 0.1 (gfx950)"; }
 
 static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, int64_t inner,
+                       int64_t stride, int64_t dist, int64_t outer, int64_t dist_outer);
+
+// A C2C length with a tiled kernel in some layout.
+static bool has_tiled_c2c(int bp, int64_t m) {
+  return find_kernel(bp, KIND_C2C, m, false) != nullptr ||
+         find_kernel(bp, KIND_C2C, m, true) != nullptr;
+}
+
+// Sub-plans, scratch and twiddles of a long plan (see LongPlan).
+static int make_long(pfx_plan p) {
+  const int bp = base_prec(p->precision);
+  const size_t esz = bp ? 16 : 8;
+  LongPlan *lp = new (std::nothrow) LongPlan;
+  if (!lp) return fail(PFX_ERR_ALLOC, "out of host memory");
+  p->lp = lp;
+  const int64_t n = p->length, B = p->inner, st = p->stride, d = p->dist;
+  int rc = PFX_SUCCESS;
+  if (p->kind == PFX_C2C) {
+    // the most even split into two tiled lengths (n1 <= n2)
+    for (int64_t f = 2; f * f <= n; ++f)
+      if (n % f == 0 && has_tiled_c2c(bp, f) && has_tiled_c2c(bp, n / f)) lp->n1 = f;
+    if (!lp->n1) return fail(PFX_ERR_UNSUPPORTED, "no split into two tiled lengths");
+    lp->n2 = n / lp->n1;
+    rc = plan_create(&lp->a, p->precision, PFX_C2C, lp->n2, lp->n1, lp->n1 * st, st, B, d);
+    if (rc == PFX_SUCCESS)
+      rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->n1, lp->n2, st, lp->n1 * st, B, d);
+    if (rc != PFX_SUCCESS) return rc;
+    if (B > 0) {
+      const int64_t span = (B - 1) * d + (n - 1) * st + 1;
+      PFX_HIP(hipMalloc(&lp->scratch, (size_t)span * esz), "hipMalloc long-plan scratch");
+    }
+  } else {
+    if (n % 2) return fail(PFX_ERR_UNSUPPORTED, "odd real length beyond the direct-DFT limit");
+    rc = plan_create(&lp->c, p->precision, PFX_C2C, n / 2, B, 1, d / 2, p->outer,
+                     p->dist_outer / 2);
+    if (rc != PFX_SUCCESS) return rc;
+  }
+  PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc long-plan twiddles");
+  PFX_HIP(make_twiddles(bp, n, lp->tw, nullptr), "long-plan twiddles");
+  PFX_HIP(hipStreamSynchronize(nullptr), "long-plan twiddles");
+  return PFX_SUCCESS;
+}
+
+static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, int64_t inner,
                        int64_t stride, int64_t dist, int64_t outer, int64_t dist_outer) {
   if (!out) return fail(PFX_ERR_INVALID_ARG, "plan pointer is null");
   *out = nullptr;
@@ -305,13 +368,16 @@ static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, i
       p->xcd_remap = order;
     }
   }
-  if (!p->kernel) {
-    if (length > naive_max_length(bp)) {
-      delete p;
-      return fail(PFX_ERR_UNSUPPORTED,
-                  "length %lld has no tiled kernel and exceeds the direct-DFT fallback "
-                  "limit of %d", (long long)length, naive_max_length(bp));
+  if (!p->kernel && length > naive_max_length(bp)) {
+    const int rc = make_long(p);
+    if (rc != PFX_SUCCESS) {
+      const std::string msg = g_last_error;
+      pfx_plan_destroy(p);
+      return fail(rc, "length %lld has no tiled kernel and exceeds the direct-DFT fallback "
+                      "limit of %d: %s", (long long)length, naive_max_length(bp), msg.c_str());
     }
+    *out = p;
+    return PFX_SUCCESS;
   }
   if (p->kernel) {
     const int64_t blocks = (total + p->kernel->per_block - 1) / p->kernel->per_block;
@@ -408,6 +474,78 @@ int pfx_plan_create_c2r(pfx_plan *plan, int precision, int64_t length, int64_t b
 static int execute_impl(pfx_plan p, int direction, void *data, double scale, pfx_stream stream,
                         void *real_oop, int64_t real_dist);
 
+// A C2C tiled plan from `src` into `dst` at the same offsets (src null: in
+// place on dst), scale 1.
+static int launch_tiled(pfx_plan p, int direction, void *dst, const void *src, hipStream_t s) {
+  if (!p->kernel) return fail(PFX_ERR_INTERNAL, "sub-plan without a tiled kernel");
+  StockhamArgs a;
+  a.data = dst;
+  a.src = src;
+  a.src_dist = p->dist;
+  a.src_dist_outer = p->dist_outer;
+  a.tw = p->tw;
+  a.ptw = p->ptw;
+  a.total = p->inner * p->outer;
+  a.inner = p->inner;
+  a.stride = p->stride;
+  a.dist = p->dist;
+  a.dist_outer = p->dist_outer;
+  a.scale = 1.0;
+  a.use_scale = 0;
+  a.backward = direction == PFX_BACKWARD;
+  a.xcd_remap = p->xcd_remap;
+  a.in_seg = a.out_seg = 0;
+  if (a.total == 0) return PFX_SUCCESS;
+  p->kernel->launch(a, s);
+  PFX_HIP(hipGetLastError(), "kernel launch");
+  return PFX_SUCCESS;
+}
+
+// The steps of a long plan (see LongPlan), on one stream.  The scratch is
+// the plan's own: executions of one long plan must not overlap.
+static int run_long(pfx_plan p, int direction, void *data, double scale, hipStream_t s) {
+  LongPlan *lp = p->lp;
+  const int bp = base_prec(p->precision);
+  const size_t esz = bp ? 16 : 8;
+  if (p->kind == PFX_C2C) {
+    FourStepArgs fa;
+    fa.n1 = lp->n1;
+    fa.n2 = lp->n2;
+    fa.stride = p->stride;
+    fa.batch = p->inner;
+    fa.dist = p->dist;
+    fa.backward = direction == PFX_BACKWARD;
+    for (int64_t o = 0; o < p->outer; ++o) {
+      char *base = static_cast<char *>(data) + (size_t)(o * p->dist_outer) * esz;
+      int rc = launch_tiled(lp->a, direction, lp->scratch, base, s);
+      if (rc != PFX_SUCCESS) return rc;
+      fa.use_scale = 0;
+      fa.scale = 1.0;
+      PFX_HIP(launch_fourstep_twiddle(bp, lp->scratch, lp->tw, fa, s), "four-step twiddle");
+      rc = launch_tiled(lp->c, direction, lp->scratch, nullptr, s);
+      if (rc != PFX_SUCCESS) return rc;
+      fa.use_scale = scale != 1.0;
+      fa.scale = scale;
+      PFX_HIP(launch_fourstep_transpose(bp, lp->scratch, base, fa, s), "four-step transpose");
+    }
+    return PFX_SUCCESS;
+  }
+  RealPassArgs ra;
+  ra.h = p->length / 2;
+  ra.batch = p->inner;
+  ra.dist = p->dist / 2;
+  ra.use_scale = scale != 1.0;
+  ra.scale = scale;
+  if (p->kind == PFX_R2C) {
+    int rc = pfx_execute_scaled(lp->c, PFX_FORWARD, data, 1.0, s);
+    if (rc != PFX_SUCCESS) return rc;
+    PFX_HIP(launch_real_post(bp, data, lp->tw, ra, s), "long R2C post-pass");
+    return PFX_SUCCESS;
+  }
+  PFX_HIP(launch_real_pre(bp, data, lp->tw, ra, s), "long C2R pre-pass");
+  return pfx_execute_scaled(lp->c, PFX_BACKWARD, data, 1.0, s);
+}
+
 int pfx_execute_scaled(pfx_plan p, int direction, void *data, double scale, pfx_stream stream) {
   return execute_impl(p, direction, data, scale, stream, nullptr, 0);
 }
@@ -449,6 +587,7 @@ static int execute_impl(pfx_plan p, int direction, void *data, double scale, pfx
                   (long long)real_dist);
   }
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (p->lp) return run_long(p, direction, data, scale, s);
   const bool real = p->kind != PFX_C2C;
   if (p->kernel) {
     StockhamArgs a;
@@ -591,6 +730,16 @@ int pfx_execute(pfx_plan plan, int direction, void *data, pfx_stream stream) {
 
 int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
   if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (p->lp && flag != PFX_PLAN_ESTIMATE) {
+    // the sub-plans: `a` in place on the caller's scratch (its footprint is
+    // the plan's), `c` on the plan's own scratch (C2C) or the caller's rows
+    if (!scratch) return fail(PFX_ERR_INVALID_ARG, "tuning needs a scratch buffer");
+    if (p->lp->a) {
+      int rc = pfx_plan_tune(p->lp->a, flag, scratch, stream);
+      if (rc != PFX_SUCCESS) return rc;
+    }
+    return pfx_plan_tune(p->lp->c, flag, p->kind == PFX_C2C ? p->lp->scratch : scratch, stream);
+  }
   if (flag == PFX_PLAN_ESTIMATE || !p->kernel) return PFX_SUCCESS;
   if (flag != PFX_PLAN_MEASURE && flag != PFX_PLAN_PATIENT)
     return fail(PFX_ERR_INVALID_ARG, "unknown planning flag %d", flag);
@@ -727,6 +876,13 @@ int pfx_plan_destroy(pfx_plan p) {
   if (!p) return PFX_SUCCESS;
   if (p->tw) hipFree(p->tw);
   if (p->ptw) hipFree(p->ptw);
+  if (p->lp) {
+    pfx_plan_destroy(p->lp->a);
+    pfx_plan_destroy(p->lp->c);
+    if (p->lp->scratch) hipFree(p->lp->scratch);
+    if (p->lp->tw) hipFree(p->lp->tw);
+    delete p->lp;
+  }
   delete p;
   return PFX_SUCCESS;
 }
@@ -742,6 +898,17 @@ int pfx_plan_describe(pfx_plan p, char *name, size_t name_len, int *per_block, i
              k->col ? "col" : "row", k->n, k->radices, k->per_block, k->tpt);
     if (per_block) *per_block = k->per_block;
     if (threads) *threads = k->tpt * k->per_block;
+  } else if (p->lp && p->kind == PFX_C2C) {
+    snprintf(buf, sizeof(buf), "%s_c2c_fourstep_n%lld_%lldx%lld",
+             p->precision == PFX_F64 ? "f64" : "f32", (long long)p->length,
+             (long long)p->lp->n1, (long long)p->lp->n2);
+    if (per_block) *per_block = 1;
+    if (threads) *threads = 256;
+  } else if (p->lp) {
+    snprintf(buf, sizeof(buf), "%s_%s_long_n%lld", p->precision == PFX_F64 ? "f64" : "f32",
+             p->kind == PFX_R2C ? "r2c" : "c2r", (long long)p->length);
+    if (per_block) *per_block = 1;
+    if (threads) *threads = 256;
   } else {
     snprintf(buf, sizeof(buf), "%s_direct_dft_n%lld", p->precision == PFX_F64 ? "f64" : "f32",
              (long long)p->length);

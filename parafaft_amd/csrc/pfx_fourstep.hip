@@ -1,0 +1,239 @@
+// pfx_fourstep.hip -- the element-wise passes of the four-step plan for long
+// lengths (n = n1 * n2, both factors with tiled kernels) and of the long R2C
+// / C2R plans built on it.
+//
+// Four-step, for element j = j1 + n1 j2 of a transform (j1 < n1, j2 < n2)
+// and output k = n2 k1 + k2:
+//   X[n2 k1 + k2] = sum_j1 W_n1^(j1 k1) W_n^(j1 k2) sum_j2 x[j1 + n1 j2] W_n2^(j2 k2)
+// 1. length-n2 DFTs along j2 (element stride n1 s), out of place into the
+//    plan's scratch at the same offsets: Y[j1][k2] at j1 + n1 k2;
+// 2. the twiddle Y[j1][k2] *= W_n^(j1 k2) (this file);
+// 3. length-n1 DFTs along j1 (element stride s), in place on the scratch:
+//    Z[k1][k2] at k1 + n1 k2;
+// 4. the transpose into the caller's array: X at n2 k1 + k2 (this file),
+//    with the caller's scale.
+// Positions are in elements of the transform, times the stride s, plus b
+// dist for transform b of the batch.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include "pfx_internal.hpp"
+
+namespace pfx {
+
+namespace {
+
+// Step 2.  s == 1: j1 fastest across threads (contiguous); else the batch
+// index fastest (contiguous when dist == 1, the batch-contiguous layout).
+template <typename T>
+__global__ void __launch_bounds__(256) fourstep_twiddle_kernel(V2<T>* buf, const V2<T>* tw,
+                                                               FourStepArgs a) {
+  const int64_t n = a.n1 * a.n2, total = n * a.batch;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    int64_t b, j1, k2;
+    if (a.stride == 1) {
+      j1 = t % a.n1;
+      const int64_t r = t / a.n1;
+      k2 = r % a.n2;
+      b = r / a.n2;
+    } else {
+      b = t % a.batch;
+      const int64_t r = t / a.batch;
+      j1 = r % a.n1;
+      k2 = r / a.n1;
+    }
+    V2<T>* p = buf + b * a.dist + (j1 + a.n1 * k2) * a.stride;
+    const V2<T> w = tw[j1 * k2];  // j1 k2 <= (n1 - 1)(n2 - 1) < n
+    *p = a.backward ? cmulc<T>(*p, w) : cmul<T>(*p, w);
+  }
+}
+
+// Step 4, rows (s == 1): per transform an n2 x n1 matrix (row k2, column k1)
+// to n1 x n2, through 32 x 32 LDS tiles so both sides are coalesced.
+template <typename T>
+__global__ void __launch_bounds__(256) fourstep_transpose_rows(const V2<T>* src, V2<T>* dst,
+                                                               FourStepArgs a) {
+  __shared__ V2<T> tile[32][33];
+  const int64_t t1 = (a.n1 + 31) / 32, t2 = (a.n2 + 31) / 32;
+  const int64_t nt = t1 * t2 * a.batch;
+  const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+  const T sc = T(a.scale);
+  for (int64_t blk = blockIdx.x; blk < nt; blk += gridDim.x) {
+    const int64_t b = blk / (t1 * t2), r = blk - b * (t1 * t2);
+    const int64_t k1_0 = (r % t1) * 32, k2_0 = (r / t1) * 32;
+    const V2<T>* s = src + b * a.dist;
+    V2<T>* d = dst + b * a.dist;
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t k2 = k2_0 + i, k1 = k1_0 + tx;
+      if (k1 < a.n1 && k2 < a.n2) tile[i][tx] = s[k1 + a.n1 * k2];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+      const int64_t k1 = k1_0 + i, k2 = k2_0 + tx;
+      if (k1 < a.n1 && k2 < a.n2) {
+        V2<T> v = tile[tx][i];
+        if (a.use_scale) v = scl<T>(v, sc);
+        d[a.n2 * k1 + k2] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// Step 4, strided transforms: one element per thread, the batch index
+// fastest (contiguous when dist == 1).
+template <typename T>
+__global__ void __launch_bounds__(256) fourstep_transpose_strided(const V2<T>* src, V2<T>* dst,
+                                                                  FourStepArgs a) {
+  const int64_t n = a.n1 * a.n2, total = n * a.batch;
+  const T sc = T(a.scale);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t % a.batch, o = t / a.batch;  // output element o = n2 k1 + k2
+    const int64_t k1 = o / a.n2, k2 = o - k1 * a.n2;
+    V2<T> v = src[b * a.dist + (k1 + a.n1 * k2) * a.stride];
+    if (a.use_scale) v = scl<T>(v, sc);
+    dst[b * a.dist + o * a.stride] = v;
+  }
+}
+
+// Long R2C, after the length-h complex FFT of each padded row read as h
+// complex pairs z[m] = x[2m] + i x[2m+1]: pairs (k, h - k), k <= h / 2,
+//   X[k] = (s - i W_2h^k d) / 2,  s = Z[k] + conj Z[h-k],  d = Z[k] - conj Z[h-k]
+// (Z[h] = Z[0]); X[h] goes to the row's spare slot h.  tw: W_2h^k, k <= h.
+template <typename T>
+__global__ void __launch_bounds__(256) real_post_kernel(V2<T>* data, const V2<T>* tw,
+                                                        RealPassArgs a) {
+  const int64_t np = a.h / 2 + 1, total = np * a.batch;
+  const T half = T(0.5), sc = T(a.scale);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / np, k = t - b * np;
+    V2<T>* row = data + b * a.dist;
+    const int64_t km = (a.h - k) % a.h;
+    const V2<T> A = row[k], Bm = row[km];
+    {
+      const V2<T> B = conj<T>(Bm);
+      const V2<T> s = add<T>(A, B), d = sub<T>(A, B);
+      const V2<T> wd = cmul<T>(tw[k], d);
+      V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
+      if (a.use_scale) x = scl<T>(x, sc);
+      row[k] = x;
+    }
+    if (2 * k != a.h) {  // X[h - k]; k = 0 gives X[h]
+      const int64_t kk = a.h - k;
+      const V2<T> B2 = conj<T>(A);
+      const V2<T> s = add<T>(Bm, B2), d = sub<T>(Bm, B2);
+      const V2<T> wd = cmul<T>(tw[kk], d);
+      V2<T> x = mk<T>(half * (s.x + wd.y), half * (s.y - wd.x));
+      if (a.use_scale) x = scl<T>(x, sc);
+      row[kk] = x;
+    }
+  }
+}
+
+// Long C2R, before the inverse length-h complex FFT: from the half spectrum
+// X[0..h] (Im X[0] and Im X[h] ignored, a Hermitian input),
+//   Z[k] = s + i W_2h^-k d,  s = X[k] + conj X[h-k],  d = X[k] - conj X[h-k],
+// for k < h; the unnormalised inverse then gives z[m] = x[2m] + i x[2m+1].
+template <typename T>
+__global__ void __launch_bounds__(256) real_pre_kernel(V2<T>* data, const V2<T>* tw,
+                                                       RealPassArgs a) {
+  const int64_t np = a.h / 2 + 1, total = np * a.batch;
+  const T sc = T(a.scale);
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
+       t += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = t / np, k = t - b * np;
+    V2<T>* row = data + b * a.dist;
+    const int64_t km = a.h - k;
+    V2<T> Xk = row[k], Xm = row[km];
+    if (k == 0) {
+      Xk.y = T(0);
+      Xm.y = T(0);
+    }
+    {
+      const V2<T> B = conj<T>(Xm);
+      const V2<T> s = add<T>(Xk, B), d = sub<T>(Xk, B);
+      const V2<T> wd = cmulc<T>(d, tw[k]);  // W^-k d
+      V2<T> z = mk<T>(s.x - wd.y, s.y + wd.x);
+      if (a.use_scale) z = scl<T>(z, sc);
+      row[k] = z;
+    }
+    if (k != 0 && 2 * k != a.h) {
+      const V2<T> B = conj<T>(Xk);
+      const V2<T> s = add<T>(Xm, B), d = sub<T>(Xm, B);
+      const V2<T> wd = cmulc<T>(d, tw[km]);
+      V2<T> z = mk<T>(s.x - wd.y, s.y + wd.x);
+      if (a.use_scale) z = scl<T>(z, sc);
+      row[km] = z;
+    }
+  }
+}
+
+int grid_for(int64_t work) {
+  const int64_t blocks = (work + 255) / 256;
+  return (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
+}
+
+}  // namespace
+
+hipError_t launch_fourstep_twiddle(int prec, void* buf, const void* tw, const FourStepArgs& a,
+                                   hipStream_t s) {
+  const int g = grid_for(a.n1 * a.n2 * a.batch);
+  if (prec)
+    fourstep_twiddle_kernel<double><<<g, 256, 0, s>>>(static_cast<V2<double>*>(buf),
+                                                      static_cast<const V2<double>*>(tw), a);
+  else
+    fourstep_twiddle_kernel<float><<<g, 256, 0, s>>>(static_cast<V2<float>*>(buf),
+                                                     static_cast<const V2<float>*>(tw), a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fourstep_transpose(int prec, const void* src, void* dst, const FourStepArgs& a,
+                                     hipStream_t s) {
+  if (a.stride == 1) {
+    const int64_t tiles = ((a.n1 + 31) / 32) * ((a.n2 + 31) / 32) * a.batch;
+    const int g = (int)(tiles < 65536 ? tiles : 65536);
+    if (prec)
+      fourstep_transpose_rows<double><<<g, 256, 0, s>>>(static_cast<const V2<double>*>(src),
+                                                        static_cast<V2<double>*>(dst), a);
+    else
+      fourstep_transpose_rows<float><<<g, 256, 0, s>>>(static_cast<const V2<float>*>(src),
+                                                       static_cast<V2<float>*>(dst), a);
+  } else {
+    const int g = grid_for(a.n1 * a.n2 * a.batch);
+    if (prec)
+      fourstep_transpose_strided<double><<<g, 256, 0, s>>>(static_cast<const V2<double>*>(src),
+                                                           static_cast<V2<double>*>(dst), a);
+    else
+      fourstep_transpose_strided<float><<<g, 256, 0, s>>>(static_cast<const V2<float>*>(src),
+                                                          static_cast<V2<float>*>(dst), a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_real_post(int prec, void* data, const void* tw, const RealPassArgs& a,
+                            hipStream_t s) {
+  const int g = grid_for((a.h / 2 + 1) * a.batch);
+  if (prec)
+    real_post_kernel<double><<<g, 256, 0, s>>>(static_cast<V2<double>*>(data),
+                                               static_cast<const V2<double>*>(tw), a);
+  else
+    real_post_kernel<float><<<g, 256, 0, s>>>(static_cast<V2<float>*>(data),
+                                              static_cast<const V2<float>*>(tw), a);
+  return hipGetLastError();
+}
+
+hipError_t launch_real_pre(int prec, void* data, const void* tw, const RealPassArgs& a,
+                           hipStream_t s) {
+  const int g = grid_for((a.h / 2 + 1) * a.batch);
+  if (prec)
+    real_pre_kernel<double><<<g, 256, 0, s>>>(static_cast<V2<double>*>(data),
+                                              static_cast<const V2<double>*>(tw), a);
+  else
+    real_pre_kernel<float><<<g, 256, 0, s>>>(static_cast<V2<float>*>(data),
+                                             static_cast<const V2<float>*>(tw), a);
+  return hipGetLastError();
+}
+
+}  // namespace pfx

@@ -131,4 +131,30 @@ hipError_t launch_copy2d(void *dst, size_t dpitch, const void *src, size_t spitc
                          size_t width, size_t height, hipStream_t s);
 hipError_t launch_scale(void *data, int64_t count, int prec, double s, hipStream_t st);
 
+// Element-wise passes of the four-step plan for long lengths (pfx_fourstep.hip):
+// transform b of `batch` at b * dist, element j at j * stride, n = n1 * n2.
+struct FourStepArgs {
+  int64_t n1, n2, stride, batch, dist;
+  int backward;
+  int use_scale;
+  double scale;
+};
+// buf[j1 + n1 k2] *= W_n^(+-j1 k2), tw: W_n^i, i < n
+hipError_t launch_fourstep_twiddle(int prec, void *buf, const void *tw, const FourStepArgs &a,
+                                   hipStream_t s);
+// dst[n2 k1 + k2] = scale * src[k1 + n1 k2]
+hipError_t launch_fourstep_transpose(int prec, const void *src, void *dst,
+                                     const FourStepArgs &a, hipStream_t s);
+// Long R2C post-pass / C2R pre-pass on padded rows (row b at b * dist complex
+// slots, h + 1 slots each); tw: W_2h^k, k <= h.
+struct RealPassArgs {
+  int64_t h, batch, dist;
+  int use_scale;
+  double scale;
+};
+hipError_t launch_real_post(int prec, void *data, const void *tw, const RealPassArgs &a,
+                            hipStream_t s);
+hipError_t launch_real_pre(int prec, void *data, const void *tw, const RealPassArgs &a,
+                           hipStream_t s);
+
 }  // namespace pfx

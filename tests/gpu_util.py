@@ -36,10 +36,12 @@ def tol(prec: str, n: int) -> float:
 
 
 # Which lengths the library must accept: every product of a radix plan in
-# pfx_plans.inc has tiled kernels (C2C, and R2C/C2R of twice the length), and
-# any length up to the direct-DFT limit (2n complex values in 64 KiB of LDS)
-# runs the fallback.  A plan outside both must be refused; every other plan
-# must be created (a refusal there is a failure, not a skip).
+# pfx_plans.inc has tiled kernels (C2C, and R2C/C2R of twice the length), any
+# length up to the direct-DFT limit (2n complex values in 64 KiB of LDS) runs
+# the fallback, and a longer length that splits into two tiled lengths runs
+# the four-step plan (an even real length: its half as C2C).  A plan outside
+# these must be refused; every other plan must be created (a refusal there is
+# a failure, not a skip).
 NAIVE_MAX = {"f32": 4096, "f64": 2048}
 
 
@@ -57,10 +59,25 @@ def tiled_lengths():
     return out
 
 
+def long_split(n: int):
+    """The four-step split of a long length: the most even n1 * n2 = n with
+    both factors tiled (n1 <= n2), or None."""
+    t = tiled_lengths()
+    best = None
+    f = 2
+    while f * f <= n:
+        if n % f == 0 and f in t and n // f in t:
+            best = (f, n // f)
+        f += 1
+    return best
+
+
 def c2c_supported(prec: str, n: int) -> bool:
-    return n in tiled_lengths() or n <= NAIVE_MAX["f64" if prec == "f64" else "f32"]
+    return n in tiled_lengths() or n <= NAIVE_MAX["f64" if prec == "f64" else "f32"] or \
+        long_split(n) is not None
 
 
 def real_supported(prec: str, n: int) -> bool:
     return (n % 2 == 0 and n // 2 in tiled_lengths()) or \
-        n <= NAIVE_MAX["f64" if prec == "f64" else "f32"]
+        n <= NAIVE_MAX["f64" if prec == "f64" else "f32"] or \
+        (n % 2 == 0 and c2c_supported(prec, n // 2))

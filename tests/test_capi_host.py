@@ -109,12 +109,14 @@ def test_batch_product_overflow_rejected():
     assert b"overflows" in L.pfx_last_error_message()
 
 
-@pytest.mark.parametrize("prec,n", [(0, 8192), (0, 4097), (1, 3072), (1, 4096 + 2)])
+@pytest.mark.parametrize("prec,n", [(0, 8191), (0, 4097), (1, 2 * 2053), (1, 4096 + 2),
+                                    (0, 3 * 4099)])
 def test_lengths_without_a_kernel_are_refused_on_the_host(prec, n):
     """C2C lengths with no tiled plan beyond the direct-DFT limit (4096
-    complex<float>, 2048 complex<double>) are refused before any device
-    work, with PFX_ERR_UNSUPPORTED; the GPU tests hold the complementary
-    rule (every other length must plan)."""
+    complex<float>, 2048 complex<double>) that do not split into two tiled
+    lengths either are refused before any device work, with
+    PFX_ERR_UNSUPPORTED; the GPU tests hold the complementary rule (every
+    other length must plan)."""
     import sys
     sys.path.insert(0, ROOT)
     from tests.gpu_util import c2c_supported
@@ -125,6 +127,21 @@ def test_lengths_without_a_kernel_are_refused_on_the_host(prec, n):
     assert L.pfx_plan_create_c2c(C.byref(h), prec, n, 4, 1, n) == 2
     assert h.value is None
     assert b"fallback" in L.pfx_last_error_message()
+    assert b"no split into two tiled lengths" in L.pfx_last_error_message()
+
+
+def test_long_split_rule():
+    """The four-step split the tests expect: the most even pair of tiled
+    factors (the library's rule in make_long)."""
+    import sys
+    sys.path.insert(0, ROOT)
+    from tests.gpu_util import c2c_supported, long_split, real_supported
+    assert long_split(8192) == (64, 128)
+    assert long_split(16384) == (128, 128)
+    assert long_split(6144) == (64, 96)
+    assert long_split(8191) is None and long_split(4097) is None
+    assert c2c_supported("f64", 3072) and c2c_supported("f32", 1 << 20)
+    assert real_supported("f32", 32768) and not real_supported("f32", 8194)
 
 
 def test_tune_cache_load_counts_dropped_lines(tmp_path):

@@ -1,0 +1,205 @@
+"""Lengths beyond one kernel: the four-step C2C plan (n = n1 * n2, both
+factors tiled) and the long R2C / C2R plans built on it, against the oracle.
+
+hipFFT, which the backend replaces (backend/hipfft/fft_backend_hipfft.hpp:
+294-314), plans any length; here a length with no tiled kernel beyond the
+direct-DFT limit runs as: length-n2 transforms along j2 into a scratch
+buffer, the twiddle W_n^(j1 k2), length-n1 transforms along j1, and a
+transpose back (pfx_fourstep.hip).  Checked in rows, batch-contiguous
+columns, general strides with gaps (every element the plan does not own
+comes back bit-identical) and two-level batches, both directions, with a
+fused scale; the FP32 tolerance is the tiled plans' 1e-6 log2 N."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import long_split, rel_l2, to_dev, to_host, tol
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (prec, n, layout)
+    ("f32", 8192, "row"), ("f32", 8192, "col"), ("f64", 8192, "row"),
+    ("f32", 16384, "gen"), ("f64", 16384, "col"), ("f32", 6144, "row"),
+    ("f64", 3072, "gen"), ("f32", 65536, "row"), ("f32", 12288, "col"),
+    ("f64", 10240, "row"), ("f32", 1 << 20, "row"),
+]
+
+
+def _geometry(n, layout):
+    if layout == "row":
+        return 3, 1, n + 5                       # batch, stride, dist (gaps)
+    if layout == "col":
+        return 5, 5, 1
+    return 2, 3, 3 * (n - 1) + 1 + 7               # general strided rows with gaps
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: "-".join(map(str, c)))
+def test_long_c2c_vs_oracle(case):
+    import parafaft_amd as P
+    prec, n, layout = case
+    assert long_split(n) is not None
+    batch, stride, dist = _geometry(n, layout)
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    size = (batch - 1) * dist + (n - 1) * stride + 1
+    x = O.uniform_complex((size,), seed=n + len(layout)).astype(cdt)
+    owned = np.zeros(size, bool)
+    idx = (np.arange(batch)[:, None] * dist + np.arange(n)[None, :] * stride).ravel()
+    owned[idx] = True
+    plan = P.Plan.c2c(prec, n, batch, stride, dist)
+    n1, n2 = long_split(n)
+    assert plan.describe()[0].endswith(f"fourstep_n{n}_{n1}x{n2}"), plan.describe()
+    for direction, sign, scale in ((0, -1, 1.0), (1, +1, 0.5)):
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, stride, dist, sign)
+        ref *= scale
+        d = to_dev(x)
+        plan.execute(direction, d, scale=scale)
+        got = to_host(d)
+        e = rel_l2(got[owned], ref[owned])
+        assert e <= tol(prec, n), (direction, e)
+        assert np.array_equal(got[~owned].view(np.uint8), x[~owned].view(np.uint8))
+        del ref, got, d
+
+
+def test_long_c2c_two_level_batch_and_roundtrip():
+    """pfx_plan_create_c2c_many (outer slices run one after another on the
+    plan's scratch) and backward(forward(x)) = n x."""
+    import parafaft_amd as P
+    n, inner, outer = 8192, 3, 2
+    stride, dist, odist = 1, n, 3 * n + 11
+    size = (outer - 1) * odist + (inner - 1) * dist + n
+    x = O.uniform_complex((size,), seed=77).astype(np.complex64)
+    plan = P.Plan.c2c("f32", n, inner, stride, dist, outer, odist)
+    ref = x.astype(np.complex128).copy()
+    for o in range(outer):
+        sl = ref[o * odist:o * odist + (inner - 1) * dist + n]
+        O.c2c_batched(sl, n, inner, stride, dist, -1)
+        ref[o * odist:o * odist + sl.size] = sl
+    d = to_dev(x)
+    plan.execute(0, d)
+    got = to_host(d)
+    for o in range(outer):
+        for b in range(inner):
+            s = o * odist + b * dist
+            assert rel_l2(got[s:s + n], ref[s:s + n]) <= tol("f32", n)
+    plan.execute(1, d, scale=1.0 / n)
+    back = to_host(d)
+    for o in range(outer):
+        for b in range(inner):
+            s = o * odist + b * dist
+            assert rel_l2(back[s:s + n], x[s:s + n]) <= 2 * tol("f32", n)
+
+
+def test_long_c2c_measure_tuned():
+    """Measure planning tunes the two sub-plans; the result is unchanged."""
+    import parafaft_amd as P
+    n, batch = 16384, 4
+    x = O.uniform_complex((batch * n,), seed=5).astype(np.complex64)
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, 1, n, -1)
+    plan = P.Plan.c2c("f32", n, batch, 1, n)
+    plan.tune(1, to_dev(np.zeros(batch * n, np.complex64)))
+    d = to_dev(x)
+    plan.execute(0, d)
+    assert rel_l2(to_host(d), ref) <= tol("f32", n)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 16384), ("f64", 16384), ("f32", 12288),
+                                    ("f32", 1 << 18), ("f64", 8192 + 4096)])
+def test_long_r2c_c2r_vs_oracle(prec, n):
+    """Even real lengths whose half has no tiled kernel: the half-length
+    complex FFT of the padded rows read as pairs, then the R2C post-pass;
+    the C2R pre-pass, then the inverse."""
+    import parafaft_amd as P
+    batch = 3
+    h = n // 2 + 1
+    pd = 2 * h + 2
+    rdt = np.float32 if prec == "f32" else np.float64
+    real = O.uniform(batch * n, seed=n + 1).reshape(batch, n).astype(rdt)
+    pad = np.zeros((batch, pd), dtype=rdt)
+    pad[:, :n] = real
+    ref = pad.astype(np.float64).copy()
+    O.r2c_inplace(ref, n, batch, 1, pd)
+    r2c = P.Plan.r2c(prec, n, batch, 1, pd)
+    c2r = P.Plan.c2r(prec, n, batch, 1, pd)
+    assert r2c.describe()[0].endswith(f"r2c_long_n{n}"), r2c.describe()
+    assert c2r.describe()[0].endswith(f"c2r_long_n{n}"), c2r.describe()
+    d = to_dev(pad)
+    r2c.execute(0, d)
+    got = to_host(d)
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    refc = ref[:, :2 * h].copy().view(np.complex128)
+    assert rel_l2(got[:, :2 * h].copy().view(cdt), refc) <= tol(prec, n)
+    # C2R of the oracle's own spectrum against the oracle's C2R
+    ref2 = ref.copy()
+    O.c2r_inplace(ref2, n, batch, 1, pd)
+    d2 = to_dev(ref.astype(rdt))
+    c2r.execute(1, d2)
+    assert rel_l2(to_host(d2)[:, :n], ref2[:, :n]) <= tol(prec, n)
+    # round trip of the GPU's spectrum, scaled
+    c2r.execute(1, d, scale=1.0 / n)
+    assert rel_l2(to_host(d)[:, :n], real.astype(np.float64)) <= 2 * tol(prec, n)
+
+
+@pytest.mark.parametrize("fast", [True, False])
+def test_driver_with_a_long_axis(fast):
+    """A 3 x 8192 C2C and a 2 x 16384 R2C through the one-rank drivers (the
+    backend's create_stage_plan gets the long length)."""
+    import parafaft_amd as P
+    shape = [3, 8192]
+    N = int(np.prod(shape))
+    x = O.uniform_complex(tuple(shape), seed=9).astype(np.complex64)
+    ref = O.fftn(x.astype(np.complex128))
+    fft = P.ParaFaFT_C2C(shape, precision="f32", fast=fast)
+    d = to_dev(x)
+    fft.forward(d)
+    assert rel_l2(to_host(d), ref) <= tol("f32", N)
+    fft.backward(d, scale=1.0 / N)
+    assert rel_l2(to_host(d), x) <= 2 * tol("f32", N)
+    shape = [2, 16384]
+    N = int(np.prod(shape))
+    xr = O.uniform(N, seed=10).reshape(shape).astype(np.float32)
+    refr = O.rfftn_padded(xr.astype(np.float64))
+    fr = P.ParaFaFT_R2C(shape, precision="f32", fast=fast)
+    h = shape[-1] // 2 + 1
+    pad = np.zeros(fr.get_required_output_size(), np.float32)
+    pad[:2 * 2 * h].reshape(2, 2 * h)[:, :shape[-1]] = xr
+    dr = to_dev(pad)
+    fr.forward_in_place(dr)
+    got = to_host(dr)[:2 * 2 * h].copy().view(np.complex64).reshape(2, h)
+    assert rel_l2(got, refr) <= tol("f32", N)
+
+
+def test_long_refusals():
+    """Lengths that do not split into two tiled lengths are refused; a long
+    plan has no segmented or out-of-place real form and no persistent body;
+    in a batch it runs on its own (direct) beside the grouped launches."""
+    import ctypes as C
+    import parafaft_amd as P
+    from parafaft_amd import _capi as A
+    from parafaft_amd.batch import Batch
+    with pytest.raises(P.PfxError) as e:
+        P.Plan.c2c("f32", 8191, 2, 1, 8191)
+    assert e.value.status == 2
+    with pytest.raises(P.PfxError):
+        P.Plan.r2c("f32", 8194, 2, 1, 2 * 4098)
+    n = 8192
+    plan = P.Plan.c2c("f32", n, 2, 1, n)
+    d = to_dev(np.zeros(2 * n, np.complex64))
+    seg = (C.c_void_p * 1)(d.data_ptr())
+    st = (C.c_int * 1)(0)
+    cnt = (C.c_int * 1)(n)
+    m = A.SegMap(1, seg, st, cnt, 1)
+    rc = A.load().pfx_execute_segmented(plan.handle, 0, d.data_ptr(), C.byref(m), None, 1.0, None)
+    assert rc == A.PFX_ERR_UNSUPPORTED
+    # a batch with one long transform: grouped schedule matches the oracle
+    x0 = O.uniform_complex((n,), seed=1).astype(np.complex64)
+    x1 = O.uniform_complex((64, 64), seed=2).astype(np.complex64)
+    buf = to_dev(np.concatenate([x0, x1.ravel()]))
+    b = Batch("f32", [("c2c", (n,), buf.data_ptr()), ("c2c", (64, 64), buf.data_ptr() + n * 8)])
+    b.execute(0)
+    got = to_host(buf)
+    assert rel_l2(got[:n], O.fftn(x0.astype(np.complex128))) <= tol("f32", n)
+    assert rel_l2(got[n:].reshape(64, 64), O.fftn(x1.astype(np.complex128))) <= tol("f32", 4096)
+    with pytest.raises(P.PfxError):
+        b.execute_persistent(0)
