@@ -6,7 +6,8 @@ left on the device as seeded synthetic segments.
   the reference's loops applied to the GPU's own intermediate data;
 - the whole share transform is checked against the oracle pipeline
   (FP64 restated R2C / pack / unpack / C2C) on scaled-down grids, and against
-  the FP64 GPU path at the full 2048^3-on-4x4 share (512 x 512 x 2048);
+  the FP64 GPU path at the full 2048^3-on-4x4 share (512 x 512 x 2048); the
+  input is k^-3-filtered noise at every size;
 - the fused form (pack and unpack inside the transforms) is checked
   bit-exact against the stepwise path: final output, every peer's send
   segment and the rank's own receive segment, on 2D, 3D and 4D shares and
@@ -35,6 +36,37 @@ def filtered_noise(shape, seed):
     amp = np.zeros_like(k)
     amp[k > 0] = k[k > 0] ** -1.5
     return np.fft.irfftn(X * amp, s=shape, axes=list(range(len(shape))))
+
+
+def filtered_noise_device(shape, seed):
+    """filtered_noise at full share size, generated on the GPU (FP32): iid
+    N(0,1) from torch's generator, the library's own R2C / C2R of the share
+    as a periodic box with the k^-3/2 amplitude in between (data generation
+    only; returned as a host array)."""
+    import parafaft_amd as P
+    t = torch()
+    n0, n1, n2 = shape
+    h = n2 // 2 + 1
+    fft = P.ParaFaFT_R2C(list(shape), precision="f32", fast=True)
+    buf = t.zeros(fft.get_required_output_size(), dtype=t.float32, device="cuda")
+    g = t.Generator(device="cuda")
+    g.manual_seed(seed)
+    buf[:n0 * n1 * 2 * h].view(n0 * n1, 2 * h)[:, :n2] = \
+        t.randn((n0 * n1, n2), generator=g, device="cuda")
+    fft.forward_in_place(buf)
+    spec = t.view_as_complex(buf[:n0 * n1 * 2 * h].view(n0, n1, h, 2))
+    k0 = t.fft.fftfreq(n0, device="cuda", dtype=t.float64) * n0
+    k1 = t.fft.fftfreq(n1, device="cuda", dtype=t.float64) * n1
+    k2 = t.arange(h, device="cuda", dtype=t.float64)
+    for i in range(n0):   # one plane at a time keeps the FP64 temporaries small
+        k = t.sqrt(k0[i] ** 2 + k1[:, None] ** 2 + k2[None, :] ** 2)
+        amp = t.where(k > 0, k.clamp_min(1.0) ** -1.5, t.zeros_like(k)).to(t.float32)
+        spec[i] *= amp
+    fft.backward_in_place(buf, scale=1.0 / (n0 * n1 * n2))
+    out = buf[:n0 * n1 * 2 * h].view(n0 * n1, 2 * h)[:, :n2].cpu().numpy().reshape(shape)
+    del buf, spec
+    t.cuda.empty_cache()
+    return out
 
 
 def _fill_peers(share, seed, cdt):
@@ -171,17 +203,17 @@ def test_geometry_matches_oracle():
 
 def test_config3_full_share():
     """Rank (0,0) of a 2048^3 R2C on a 4x4 grid: real share 512 x 512 x 2048
-    (padded 2050), half spectrum 512 x 512 x 1025; T0 and T1 pack/unpack into
-    4 + 4 device segments each, checked bit-exact on the GPU's own data; the
-    FP32 result against the FP64 GPU path on the same input (that path is
-    pinned to the oracle above at <= 1e-13)."""
+    (padded 2050) of k^-3-filtered noise, half spectrum 512 x 512 x 1025; T0
+    and T1 pack/unpack into 4 + 4 device segments each, checked bit-exact on
+    the GPU's own data; the FP32 result against the FP64 GPU path on the
+    same input (that path is pinned to the oracle above at <= 1e-13)."""
     from parafaft_amd.pencil import PencilShareR2C
     G = [2048, 2048, 2048]
     share32 = PencilShareR2C(G, [4, 4], [0, 0], precision="f32", fused=False)
     assert share32.stage_shapes == [[512, 512, 2048], [512, 2048, 257], [2048, 512, 257]]
     assert len(share32.geoms) == 2 and all(len(g.send_counts) == 4 for g in share32.geoms)
-    rng = np.random.default_rng(77)
-    real = rng.standard_normal((512, 512, 2048)).astype(np.float32)
+    real = filtered_noise_device((512, 512, 2048), 77)
+    assert np.isfinite(real).all() and real.std() > 0
     _fill_peers(share32, 900, np.complex64)
     got32 = _run_stepwise(share32, real, np.float32, check_bytes=True)
     del share32
@@ -258,11 +290,11 @@ def test_fused_exchange_bit_exact(case, prec):
 
 
 def test_fused_full_share_bit_exact():
-    """The 512 x 512 x 2048 share of 2048^3 on 4x4, FP32: fused vs stepwise."""
+    """The 512 x 512 x 2048 share of 2048^3 on 4x4, k^-3-filtered noise,
+    FP32: fused vs stepwise."""
     from parafaft_amd.pencil import PencilShareR2C
     G = [2048, 2048, 2048]
-    rng = np.random.default_rng(78)
-    real = rng.standard_normal((512, 512, 2048)).astype(np.float32)
+    real = filtered_noise_device((512, 512, 2048), 78)
     ref_share = PencilShareR2C(G, [4, 4], [1, 2], precision="f32", fused=False)
     _fill_peers(ref_share, 910, np.complex64)
     want = _run_stepwise(ref_share, real, np.float32, check_bytes=False).view(np.uint8).ravel()
