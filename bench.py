@@ -729,6 +729,37 @@ def main():
     }
     L.pfx_ndplan_destroy(nd.value)
 
+    def time_parts(shs, scs, pchunk):
+        """Event-timed parts of an N-D forward given its stage handles (and
+        chunk-stage handles): axes 2+1 in chunks and axis 0, or each axis;
+        the mean over K steps."""
+        if pchunk:
+            pparts = [("axes 2+1 (chunked)", None), ("axis 0", [shs[D - 1]])]
+        else:
+            pparts = [(f"axis {D - 1 - st}", [h]) for st, h in enumerate(shs)]
+        cb = pchunk * n * n * esz
+        acc_ms = {lbl: 0.0 for lbl, _ in pparts}
+        for k in range(args.steps + 1):
+            evs = []
+            for lbl, hs in pparts:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(stream)
+                if hs is None:
+                    for c in range(n // pchunk):
+                        for h in scs:
+                            A.check(L.pfx_execute(h, A.PFX_FORWARD, x.data_ptr() + c * cb, sp),
+                                    "pfx_execute")
+                else:
+                    for h in hs:
+                        A.check(L.pfx_execute(h, A.PFX_FORWARD, x.data_ptr(), sp), "pfx_execute")
+                e1.record(stream)
+                evs.append((lbl, e0, e1))
+            torch.cuda.synchronize()
+            if k:  # the first pass is a warm-up
+                for lbl, e0, e1 in evs:
+                    acc_ms[lbl] += e0.elapsed_time(e1) / args.steps
+        return {lbl: round(v, 4) for lbl, v in acc_ms.items()}
+
     # ---- split-precision MFMA forms on the same grid and input (the metric's
     # "split vs plain"): planned the same way, checked against the same FP64
     # path, timed over the same K forwards
@@ -754,18 +785,43 @@ def main():
             sms = (time.perf_counter() - s0) * 1e3 / args.steps
             schunk = C.c_int()
             A.check(L.pfx_ndplan_chunk(nds.value, C.byref(schunk)), "pfx_ndplan_chunk")
-            kern = []
+            kern, shs = [], []
             for st in range(D):
                 h = C.c_void_p()
                 A.check(L.pfx_ndplan_stage(nds, st, C.byref(h)), "pfx_ndplan_stage")
                 kern.append(describe(h.value))
+                shs.append(h.value)
+            scs = []
+            for st in range(D - 1 if schunk.value else 0):
+                h = C.c_void_p()
+                A.check(L.pfx_ndplan_chunk_stage(nds, st, C.byref(h)), "pfx_ndplan_chunk_stage")
+                scs.append(h.value)
+            sparts = time_parts(shs, scs, schunk.value)
             split[mode] = {"ms_per_step": round(sms, 4),
                            "gflops": round(flops / (sms * 1e-3) / 1e9, 2),
                            "speed_vs_plain": round(ms_per_step / sms, 4),
                            "accuracy": sacc, "chunk_planes": schunk.value,
-                           "stage_kernels": kern}
+                           "stage_kernels": kern,
+                           "part_ms": sparts}
             log(f"[bench] {mode}: {sms:.3f} ms/step ({ms_per_step / sms:.3f}x plain)")
             L.pfx_ndplan_destroy(nds.value)
+        # What caps the split forms, from the parts measured above: both
+        # forms move the same bytes; the split column kernels hold 16
+        # columns per tile (128-byte runs; 32 columns of split operands
+        # exceed a tile's registers) against plain's 32 (256-byte runs), and
+        # the split/scale work around the matrix-core products adds VALU
+        # time to every pass (most for BF16's six products).
+        plain_parts = out["part_ms"]
+        split["ceiling"] = {
+            "plain_part_ms": plain_parts,
+            "split_minus_plain_ms": {m: {k: round(v["part_ms"].get(k, 0.0) - t, 4)
+                                         for k, t in plain_parts.items()}
+                                     for m, v in split.items()},
+            "note": ("same bytes, both HBM-bound: axis 0 costs the split form its 16-column "
+                     "(128-byte-run) column tiles against plain's 32-column (256-byte) ones, "
+                     "and axes 2+1 the VALU work of splitting and scaling around the "
+                     "matrix-core products; plain FP32 sits at the measured HBM touch "
+                     "ceiling, so no arithmetic form can pass it")}
         out["split_vs_plain"] = split
     # the other measured cases on the same run (the driver records only this
     # line): 512^3 FP64/FP32, one rank's R2C pencil share of 2048^3 on 4x4,
