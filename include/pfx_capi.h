@@ -61,8 +61,11 @@ const char *pfx_version(void);
  * (complex<double>); beyond that a four-step plan when the length splits
  * into two lengths with radix plans (n = n1 * n2, e.g. 8192 = 64 * 128, up
  * to 4096^2): two tiled passes, a twiddle pass and a transpose through a
- * scratch buffer of the plan's footprint that the plan owns (so executions
- * of one such plan must not overlap).  Other lengths: PFX_ERR_UNSUPPORTED.
+ * scratch buffer of the plan's footprint; any other length up to 2^23
+ * through Bluestein's chirp convolution (zero-padded rows of M >= 2n - 1,
+ * two M-point transforms).  These plans own their scratch, so executions
+ * of one such plan must not overlap.  Longer lengths that do not split:
+ * PFX_ERR_UNSUPPORTED.
  * Replaces: HipFFTBackend::create_stage_plan -> hipfftPlanMany(Z2Z/C2C)
  *   (backend/hipfft/fft_backend_hipfft.hpp:294-314); geometry as in
  *   FFTWBackend::create_stage_plan (backend/fftw3/fft_backend_fftw.hpp:387-424).
@@ -88,7 +91,9 @@ int pfx_plan_create_c2c_many(pfx_plan *plan, int precision, int64_t length,
  * Lengths: even lengths whose half has a radix plan run one kernel; others
  * up to the direct-DFT limit the fallback; longer even lengths the
  * half-length C2C plan (as above) of the rows read as complex pairs, with a
- * post-pass (C2R: a pre-pass).
+ * post-pass (C2R: a pre-pass); longer odd lengths a length-n C2C plan on
+ * scratch rows (the reals with zero imaginary parts; C2R: the Hermitian
+ * extension).
  * Replaces: HipFFTBackend::create_r2c_inplace_plan (fft_backend_hipfft.hpp:361-384),
  *   FFTWBackend::create_r2c_inplace_plan (fft_backend_fftw.hpp:439-469).
  */

@@ -130,22 +130,35 @@ int base_prec(int precision) {
 
 }  // namespace
 
-// A length with no single-kernel plan beyond the direct-DFT limit.
-//  C2C: four-step, n = n1 * n2 with tiled n1 and n2 (pfx_fourstep.hip has
-//  the element-wise steps): `a` transforms along j2 (length n2, element
-//  stride n1 s) from the caller's array into `scratch` at the same offsets,
-//  the twiddle W_n^(j1 k2) follows, `c` transforms along j1 (length n1) in
-//  place on the scratch, and the transpose writes n2 k1 + k2 back.
-//  R2C / C2R (even real length n = 2h): `c` is the length-h C2C plan of the
-//  padded rows read as complex pairs, with the R2C post-pass / C2R
-//  pre-pass around it.
+// A length with no single-kernel plan beyond the direct-DFT limit
+// (pfx_fourstep.hip has the element-wise passes).
+//  LONG_FOURSTEP (C2C, n = n1 * n2 with tiled n1 and n2): `a` transforms
+//    along j2 (length n2, element stride n1 s) from the caller's array into
+//    `scratch` at the same offsets, the twiddle W_n^(j1 k2) follows, `c`
+//    transforms along j1 (length n1) in place on the scratch, and the
+//    transpose writes n2 k1 + k2 back.
+//  LONG_CHIRP (C2C lengths that do not split, Bluestein): the chirp w[j] =
+//    exp(-pi i j^2 / n) premultiplies into scratch rows of M >= 2n - 1
+//    (zero-padded), `c` (length M) forward, times the filter's spectrum,
+//    `c` backward, and w postmultiplies (with 1/M and the caller's scale).
+//  LONG_REAL_HALF (even real length n = 2h): `c` is the length-h C2C plan
+//    of the padded rows read as complex pairs, with the R2C post-pass / C2R
+//    pre-pass around it.
+//  LONG_REAL_FULL (odd real length): `c` is a length-n C2C plan on scratch
+//    rows of n complex values, the real rows loaded into / stored from them.
+enum { LONG_FOURSTEP = 0, LONG_CHIRP = 1, LONG_REAL_HALF = 2, LONG_REAL_FULL = 3 };
+constexpr int64_t kChirpMaxLength = int64_t(1) << 23;  // M <= 2^24 = 4096^2 splits
 struct pfx_plan_t;
 struct LongPlan {
+  int mode = LONG_FOURSTEP;
   int64_t n1 = 0, n2 = 0;
+  int64_t M = 0;            // LONG_CHIRP: convolution length
   pfx_plan_t *a = nullptr;
   pfx_plan_t *c = nullptr;
-  void *scratch = nullptr;  // one outer slice's footprint of the caller's layout
-  void *tw = nullptr;       // C2C: W_n^i, i < n; real: W_n^k, k < n
+  void *scratch = nullptr;  // the mode's work rows (four-step: one outer
+                            // slice's footprint of the caller's layout)
+  void *tw = nullptr;       // four-step W_n^i, i < n; real halves W_n^k; chirp w
+  void *filt = nullptr;     // LONG_CHIRP: the filters' spectra, forward then backward
 };
 
 struct pfx_plan_t {
@@ -284,30 +297,66 @@ static int make_long(pfx_plan p) {
   if (!lp) return fail(PFX_ERR_ALLOC, "out of host memory");
   p->lp = lp;
   const int64_t n = p->length, B = p->inner, st = p->stride, d = p->dist;
+  const int64_t total = p->inner * p->outer;
   int rc = PFX_SUCCESS;
   if (p->kind == PFX_C2C) {
     // the most even split into two tiled lengths (n1 <= n2)
     for (int64_t f = 2; f * f <= n; ++f)
       if (n % f == 0 && has_tiled_c2c(bp, f) && has_tiled_c2c(bp, n / f)) lp->n1 = f;
-    if (!lp->n1) return fail(PFX_ERR_UNSUPPORTED, "no split into two tiled lengths");
-    lp->n2 = n / lp->n1;
-    rc = plan_create(&lp->a, p->precision, PFX_C2C, lp->n2, lp->n1, lp->n1 * st, st, B, d);
-    if (rc == PFX_SUCCESS)
-      rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->n1, lp->n2, st, lp->n1 * st, B, d);
-    if (rc != PFX_SUCCESS) return rc;
-    if (B > 0) {
-      const int64_t span = (B - 1) * d + (n - 1) * st + 1;
-      PFX_HIP(hipMalloc(&lp->scratch, (size_t)span * esz), "hipMalloc long-plan scratch");
+    if (lp->n1) {
+      lp->mode = LONG_FOURSTEP;
+      lp->n2 = n / lp->n1;
+      rc = plan_create(&lp->a, p->precision, PFX_C2C, lp->n2, lp->n1, lp->n1 * st, st, B, d);
+      if (rc == PFX_SUCCESS)
+        rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->n1, lp->n2, st, lp->n1 * st, B, d);
+      if (rc != PFX_SUCCESS) return rc;
+      if (B > 0) {
+        const int64_t span = (B - 1) * d + (n - 1) * st + 1;
+        PFX_HIP(hipMalloc(&lp->scratch, (size_t)span * esz), "hipMalloc long-plan scratch");
+      }
+      PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc long-plan twiddles");
+      PFX_HIP(make_twiddles(bp, n, lp->tw, nullptr), "long-plan twiddles");
+    } else {
+      if (n > kChirpMaxLength)
+        return fail(PFX_ERR_UNSUPPORTED, "no split into two tiled lengths, and beyond the "
+                                         "chirp-convolution limit of %lld",
+                    (long long)kChirpMaxLength);
+      lp->mode = LONG_CHIRP;
+      lp->M = 1;
+      while (lp->M < 2 * n - 1) lp->M <<= 1;
+      rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->M, total, 1, lp->M, 1, 0);
+      if (rc != PFX_SUCCESS) return rc;
+      if (total > 0)
+        PFX_HIP(hipMalloc(&lp->scratch, (size_t)(total * lp->M) * esz), "hipMalloc chirp rows");
+      PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc chirp");
+      PFX_HIP(hipMalloc(&lp->filt, (size_t)(2 * lp->M) * esz), "hipMalloc chirp filters");
+      PFX_HIP(make_chirp(bp, lp->tw, n, nullptr), "chirp");
+      PFX_HIP(make_chirp_filter(bp, lp->filt, lp->tw, n, lp->M, 0, nullptr), "chirp filter");
+      PFX_HIP(make_chirp_filter(bp, static_cast<char *>(lp->filt) + (size_t)lp->M * esz, lp->tw,
+                                n, lp->M, 1, nullptr), "chirp filter");
+      pfx_plan f = nullptr;
+      rc = plan_create(&f, p->precision, PFX_C2C, lp->M, 2, 1, lp->M, 1, 0);
+      if (rc == PFX_SUCCESS) rc = pfx_execute(f, PFX_FORWARD, lp->filt, nullptr);
+      const hipError_t e = hipStreamSynchronize(nullptr);
+      pfx_plan_destroy(f);
+      if (rc != PFX_SUCCESS) return rc;
+      PFX_HIP(e, "chirp filter spectra");
     }
-  } else {
-    if (n % 2) return fail(PFX_ERR_UNSUPPORTED, "odd real length beyond the direct-DFT limit");
+  } else if (n % 2 == 0) {
+    lp->mode = LONG_REAL_HALF;
     rc = plan_create(&lp->c, p->precision, PFX_C2C, n / 2, B, 1, d / 2, p->outer,
                      p->dist_outer / 2);
     if (rc != PFX_SUCCESS) return rc;
+    PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc long-plan twiddles");
+    PFX_HIP(make_twiddles(bp, n, lp->tw, nullptr), "long-plan twiddles");
+  } else {
+    lp->mode = LONG_REAL_FULL;
+    rc = plan_create(&lp->c, p->precision, PFX_C2C, n, total, 1, n, 1, 0);
+    if (rc != PFX_SUCCESS) return rc;
+    if (total > 0)
+      PFX_HIP(hipMalloc(&lp->scratch, (size_t)(total * n) * esz), "hipMalloc real rows");
   }
-  PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc long-plan twiddles");
-  PFX_HIP(make_twiddles(bp, n, lp->tw, nullptr), "long-plan twiddles");
-  PFX_HIP(hipStreamSynchronize(nullptr), "long-plan twiddles");
+  PFX_HIP(hipStreamSynchronize(nullptr), "long-plan tables");
   return PFX_SUCCESS;
 }
 
@@ -507,6 +556,41 @@ static int run_long(pfx_plan p, int direction, void *data, double scale, hipStre
   LongPlan *lp = p->lp;
   const int bp = base_prec(p->precision);
   const size_t esz = bp ? 16 : 8;
+  if (lp->mode == LONG_CHIRP) {
+    ChirpArgs ca;
+    ca.n = p->length;
+    ca.M = lp->M;
+    ca.total = p->inner * p->outer;
+    ca.inner = p->inner;
+    ca.dist = p->dist;
+    ca.dist_outer = p->dist_outer;
+    ca.stride = p->stride;
+    ca.backward = direction == PFX_BACKWARD;
+    ca.scale = scale;
+    PFX_HIP(launch_chirp_pre(bp, data, lp->scratch, lp->tw, ca, s), "chirp premultiply");
+    int rc = pfx_execute_scaled(lp->c, PFX_FORWARD, lp->scratch, 1.0, s);
+    if (rc != PFX_SUCCESS) return rc;
+    const char *B = static_cast<const char *>(lp->filt) + (ca.backward ? (size_t)lp->M * esz : 0);
+    PFX_HIP(launch_chirp_mul(bp, lp->scratch, B, ca, s), "chirp filter");
+    rc = pfx_execute_scaled(lp->c, PFX_BACKWARD, lp->scratch, 1.0, s);
+    if (rc != PFX_SUCCESS) return rc;
+    PFX_HIP(launch_chirp_post(bp, lp->scratch, data, lp->tw, ca, s), "chirp postmultiply");
+    return PFX_SUCCESS;
+  }
+  if (lp->mode == LONG_REAL_FULL) {
+    RealFullArgs fa;
+    fa.n = p->length;
+    fa.batch = p->inner;
+    fa.dist = p->dist;
+    fa.use_scale = scale != 1.0;
+    fa.scale = scale;
+    const bool r2c = p->kind == PFX_R2C;
+    PFX_HIP(launch_real_full(bp, r2c ? 0 : 2, data, lp->scratch, fa, s), "real rows load");
+    int rc = pfx_execute_scaled(lp->c, r2c ? PFX_FORWARD : PFX_BACKWARD, lp->scratch, 1.0, s);
+    if (rc != PFX_SUCCESS) return rc;
+    PFX_HIP(launch_real_full(bp, r2c ? 1 : 3, data, lp->scratch, fa, s), "real rows store");
+    return PFX_SUCCESS;
+  }
   if (p->kind == PFX_C2C) {
     FourStepArgs fa;
     fa.n1 = lp->n1;
@@ -738,7 +822,11 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
       int rc = pfx_plan_tune(p->lp->a, flag, scratch, stream);
       if (rc != PFX_SUCCESS) return rc;
     }
-    return pfx_plan_tune(p->lp->c, flag, p->kind == PFX_C2C ? p->lp->scratch : scratch, stream);
+    // `c` works on the plan's own rows except for the real halves (the
+    // caller's rows read as complex pairs)
+    if (p->lp->mode != LONG_REAL_HALF && !p->lp->scratch) return PFX_SUCCESS;  // empty batch
+    return pfx_plan_tune(p->lp->c, flag,
+                         p->lp->mode == LONG_REAL_HALF ? scratch : p->lp->scratch, stream);
   }
   if (flag == PFX_PLAN_ESTIMATE || !p->kernel) return PFX_SUCCESS;
   if (flag != PFX_PLAN_MEASURE && flag != PFX_PLAN_PATIENT)
@@ -881,6 +969,7 @@ int pfx_plan_destroy(pfx_plan p) {
     pfx_plan_destroy(p->lp->c);
     if (p->lp->scratch) hipFree(p->lp->scratch);
     if (p->lp->tw) hipFree(p->lp->tw);
+    if (p->lp->filt) hipFree(p->lp->filt);
     delete p->lp;
   }
   delete p;
@@ -898,6 +987,12 @@ int pfx_plan_describe(pfx_plan p, char *name, size_t name_len, int *per_block, i
              k->col ? "col" : "row", k->n, k->radices, k->per_block, k->tpt);
     if (per_block) *per_block = k->per_block;
     if (threads) *threads = k->tpt * k->per_block;
+  } else if (p->lp && p->lp->mode == LONG_CHIRP) {
+    snprintf(buf, sizeof(buf), "%s_c2c_bluestein_n%lld_m%lld",
+             p->precision == PFX_F64 ? "f64" : "f32", (long long)p->length,
+             (long long)p->lp->M);
+    if (per_block) *per_block = 1;
+    if (threads) *threads = 256;
   } else if (p->lp && p->kind == PFX_C2C) {
     snprintf(buf, sizeof(buf), "%s_c2c_fourstep_n%lld_%lldx%lld",
              p->precision == PFX_F64 ? "f64" : "f32", (long long)p->length,

@@ -1,6 +1,7 @@
-// pfx_fourstep.hip -- the element-wise passes of the four-step plan for long
-// lengths (n = n1 * n2, both factors with tiled kernels) and of the long R2C
-// / C2R plans built on it.
+// pfx_fourstep.hip -- the element-wise passes of the plans for lengths
+// beyond one kernel: the four-step plan (n = n1 * n2, both factors with
+// tiled kernels), Bluestein's chirp convolution for the lengths that do not
+// split, and the long R2C / C2R plans built on C2C.
 //
 // Four-step, for element j = j1 + n1 j2 of a transform (j1 < n1, j2 < n2)
 // and output k = n2 k1 + k2:
@@ -170,6 +171,137 @@ __global__ void __launch_bounds__(256) real_pre_kernel(V2<T>* data, const V2<T>*
   }
 }
 
+// ---------------------------------------------------------------- Bluestein
+// X[k] = w[k] sum_j (x[j] w[j]) conj(w[k - j]),  w[m] = exp(-pi i m^2 / n)
+// (backward: w -> conj w), the sum a circular convolution of length
+// M >= 2n - 1 done with M-point FFTs.  m^2 is reduced mod 2n in integers,
+// so sincospi sees an argument in [0, 2).
+template <typename T>
+__global__ void __launch_bounds__(256) chirp_kernel(V2<T>* w, int64_t n) {
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n;
+       j += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t r = (int64_t)(((uint64_t)j * (uint64_t)j) % (uint64_t)(2 * n));  // n < 2^31
+    double sn, cs;
+    sincospi((double)r / (double)n, &sn, &cs);
+    w[j] = mk<T>(T(cs), T(-sn));
+  }
+}
+
+// The convolution filter of one direction, before its FFT: b[m] and b[M - m]
+// = conj w[m] (forward) or w[m] (backward) for m < n, zero between.
+template <typename T>
+__global__ void __launch_bounds__(256) chirp_filter_kernel(V2<T>* b, const V2<T>* w, int64_t n,
+                                                           int64_t M, int backward) {
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < M;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t m = i < n ? i : (i > M - n ? M - i : -1);
+    V2<T> v = mk<T>(T(0), T(0));
+    if (m >= 0) v = backward ? w[m] : conj<T>(w[m]);
+    b[i] = v;
+  }
+}
+
+__device__ __forceinline__ int64_t xform_base(int64_t t, const ChirpArgs& a) {
+  const int64_t o = t / a.inner, i = t - o * a.inner;
+  return o * a.dist_outer + i * a.dist;
+}
+
+// scratch row t = (x_t[j] w'[j] for j < n, then zeros), M per row
+template <typename T>
+__global__ void __launch_bounds__(256) chirp_pre_kernel(const V2<T>* x, V2<T>* scr,
+                                                        const V2<T>* w, ChirpArgs a) {
+  const int64_t total = a.total * a.M;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = e / a.M, j = e - t * a.M;
+    V2<T> v = mk<T>(T(0), T(0));
+    if (j < a.n) {
+      const V2<T> xv = x[xform_base(t, a) + j * a.stride];
+      v = a.backward ? cmulc<T>(xv, w[j]) : cmul<T>(xv, w[j]);
+    }
+    scr[e] = v;
+  }
+}
+
+// scratch row t *= B (the filter's spectrum)
+template <typename T>
+__global__ void __launch_bounds__(256) chirp_mul_kernel(V2<T>* scr, const V2<T>* B, ChirpArgs a) {
+  const int64_t total = a.total * a.M;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x)
+    scr[e] = cmul<T>(scr[e], B[e % a.M]);
+}
+
+// x_t[k] = scale / M * w'[k] * scratch row t [k], k < n
+template <typename T>
+__global__ void __launch_bounds__(256) chirp_post_kernel(const V2<T>* scr, V2<T>* x,
+                                                         const V2<T>* w, ChirpArgs a) {
+  const int64_t total = a.total * a.n;
+  const T sc = T(a.scale / (double)a.M);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t t = e / a.n, k = e - t * a.n;
+    const V2<T> s = scr[t * a.M + k];
+    const V2<T> v = a.backward ? cmulc<T>(s, w[k]) : cmul<T>(s, w[k]);
+    x[xform_base(t, a) + k * a.stride] = scl<T>(v, sc);
+  }
+}
+
+// ------------------------------------------- real transforms through C2C
+// Odd real lengths beyond the direct-DFT limit: the whole row as a length-n
+// complex transform on a scratch of n complex values per row.
+// R2C: scratch row = (x[j], 0); after the C2C, the row takes X[k], k <= n/2.
+template <typename T>
+__global__ void __launch_bounds__(256) real_full_load(const T* rows, V2<T>* scr, RealFullArgs a) {
+  const int64_t total = a.batch * a.n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / a.n, j = e - b * a.n;
+    scr[e] = mk<T>(rows[b * a.dist + j], T(0));
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) real_full_store(const V2<T>* scr, T* rows, RealFullArgs a) {
+  const int64_t h = a.n / 2 + 1, total = a.batch * h;
+  const T sc = T(a.scale);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / h, k = e - b * h;
+    V2<T> v = scr[b * a.n + k];
+    if (a.use_scale) v = scl<T>(v, sc);
+    rows[b * a.dist + 2 * k] = v.x;
+    rows[b * a.dist + 2 * k + 1] = v.y;
+  }
+}
+// C2R: scratch row = the Hermitian extension of X[0..n/2] (Im X[0], and
+// Im X[n/2] for even n, ignored); after the inverse C2C, the row takes the
+// real parts.
+template <typename T>
+__global__ void __launch_bounds__(256) herm_full_load(const T* rows, V2<T>* scr, RealFullArgs a) {
+  const int64_t total = a.batch * a.n;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / a.n, k = e - b * a.n;
+    const bool mirror = k > a.n / 2;
+    const int64_t kk = mirror ? a.n - k : k;
+    V2<T> v = mk<T>(rows[b * a.dist + 2 * kk], rows[b * a.dist + 2 * kk + 1]);
+    if (kk == 0 || 2 * kk == a.n) v.y = T(0);
+    scr[e] = mirror ? conj<T>(v) : v;
+  }
+}
+template <typename T>
+__global__ void __launch_bounds__(256) herm_full_store(const V2<T>* scr, T* rows, RealFullArgs a) {
+  const int64_t total = a.batch * a.n;
+  const T sc = T(a.scale);
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+       e += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t b = e / a.n, j = e - b * a.n;
+    T v = scr[e].x;
+    if (a.use_scale) v *= sc;
+    rows[b * a.dist + j] = v;
+  }
+}
+
 int grid_for(int64_t work) {
   const int64_t blocks = (work + 255) / 256;
   return (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
@@ -233,6 +365,88 @@ hipError_t launch_real_pre(int prec, void* data, const void* tw, const RealPassA
   else
     real_pre_kernel<float><<<g, 256, 0, s>>>(static_cast<V2<float>*>(data),
                                              static_cast<const V2<float>*>(tw), a);
+  return hipGetLastError();
+}
+
+hipError_t make_chirp(int prec, void* w, int64_t n, hipStream_t s) {
+  if (prec) chirp_kernel<double><<<grid_for(n), 256, 0, s>>>(static_cast<V2<double>*>(w), n);
+  else chirp_kernel<float><<<grid_for(n), 256, 0, s>>>(static_cast<V2<float>*>(w), n);
+  return hipGetLastError();
+}
+
+hipError_t make_chirp_filter(int prec, void* b, const void* w, int64_t n, int64_t M, int backward,
+                             hipStream_t s) {
+  if (prec)
+    chirp_filter_kernel<double><<<grid_for(M), 256, 0, s>>>(
+        static_cast<V2<double>*>(b), static_cast<const V2<double>*>(w), n, M, backward);
+  else
+    chirp_filter_kernel<float><<<grid_for(M), 256, 0, s>>>(
+        static_cast<V2<float>*>(b), static_cast<const V2<float>*>(w), n, M, backward);
+  return hipGetLastError();
+}
+
+hipError_t launch_chirp_pre(int prec, const void* x, void* scr, const void* w, const ChirpArgs& a,
+                            hipStream_t s) {
+  const int g = grid_for(a.total * a.M);
+  if (prec)
+    chirp_pre_kernel<double><<<g, 256, 0, s>>>(static_cast<const V2<double>*>(x),
+                                               static_cast<V2<double>*>(scr),
+                                               static_cast<const V2<double>*>(w), a);
+  else
+    chirp_pre_kernel<float><<<g, 256, 0, s>>>(static_cast<const V2<float>*>(x),
+                                              static_cast<V2<float>*>(scr),
+                                              static_cast<const V2<float>*>(w), a);
+  return hipGetLastError();
+}
+
+hipError_t launch_chirp_mul(int prec, void* scr, const void* B, const ChirpArgs& a, hipStream_t s) {
+  const int g = grid_for(a.total * a.M);
+  if (prec)
+    chirp_mul_kernel<double><<<g, 256, 0, s>>>(static_cast<V2<double>*>(scr),
+                                               static_cast<const V2<double>*>(B), a);
+  else
+    chirp_mul_kernel<float><<<g, 256, 0, s>>>(static_cast<V2<float>*>(scr),
+                                              static_cast<const V2<float>*>(B), a);
+  return hipGetLastError();
+}
+
+hipError_t launch_chirp_post(int prec, const void* scr, void* x, const void* w, const ChirpArgs& a,
+                             hipStream_t s) {
+  const int g = grid_for(a.total * a.n);
+  if (prec)
+    chirp_post_kernel<double><<<g, 256, 0, s>>>(static_cast<const V2<double>*>(scr),
+                                                static_cast<V2<double>*>(x),
+                                                static_cast<const V2<double>*>(w), a);
+  else
+    chirp_post_kernel<float><<<g, 256, 0, s>>>(static_cast<const V2<float>*>(scr),
+                                               static_cast<V2<float>*>(x),
+                                               static_cast<const V2<float>*>(w), a);
+  return hipGetLastError();
+}
+
+hipError_t launch_real_full(int prec, int which, void* rows, void* scr, const RealFullArgs& a,
+                            hipStream_t s) {
+  const int64_t work = a.batch * a.n;
+  const int g = grid_for(work);
+  if (prec) {
+    double* r = static_cast<double*>(rows);
+    V2<double>* c = static_cast<V2<double>*>(scr);
+    switch (which) {
+      case 0: real_full_load<double><<<g, 256, 0, s>>>(r, c, a); break;
+      case 1: real_full_store<double><<<g, 256, 0, s>>>(c, r, a); break;
+      case 2: herm_full_load<double><<<g, 256, 0, s>>>(r, c, a); break;
+      default: herm_full_store<double><<<g, 256, 0, s>>>(c, r, a); break;
+    }
+  } else {
+    float* r = static_cast<float*>(rows);
+    V2<float>* c = static_cast<V2<float>*>(scr);
+    switch (which) {
+      case 0: real_full_load<float><<<g, 256, 0, s>>>(r, c, a); break;
+      case 1: real_full_store<float><<<g, 256, 0, s>>>(c, r, a); break;
+      case 2: herm_full_load<float><<<g, 256, 0, s>>>(r, c, a); break;
+      default: herm_full_store<float><<<g, 256, 0, s>>>(c, r, a); break;
+    }
+  }
   return hipGetLastError();
 }
 

@@ -156,5 +156,35 @@ hipError_t launch_real_post(int prec, void *data, const void *tw, const RealPass
                             hipStream_t s);
 hipError_t launch_real_pre(int prec, void *data, const void *tw, const RealPassArgs &a,
                            hipStream_t s);
+// Bluestein (pfx_fourstep.hip): transform t of `total` at
+// (t / inner) dist_outer + (t % inner) dist, element j at j * stride;
+// scratch rows of M >= 2n - 1.
+struct ChirpArgs {
+  int64_t n, M, total, inner, dist, dist_outer, stride;
+  int backward;
+  double scale;
+};
+// w[j] = exp(-pi i j^2 / n), j < n
+hipError_t make_chirp(int prec, void *w, int64_t n, hipStream_t s);
+// b[m] = b[M - m] = conj w[m] (backward: w[m]), m < n; zero between
+hipError_t make_chirp_filter(int prec, void *b, const void *w, int64_t n, int64_t M,
+                             int backward, hipStream_t s);
+hipError_t launch_chirp_pre(int prec, const void *x, void *scr, const void *w,
+                            const ChirpArgs &a, hipStream_t s);
+hipError_t launch_chirp_mul(int prec, void *scr, const void *B, const ChirpArgs &a,
+                            hipStream_t s);
+hipError_t launch_chirp_post(int prec, const void *scr, void *x, const void *w,
+                             const ChirpArgs &a, hipStream_t s);
+// Odd real lengths through a length-n C2C on scratch rows of n: which = 0
+// R2C load (reals -> complex), 1 R2C store (X[0..n/2] -> padded row), 2 C2R
+// load (Hermitian extension), 3 C2R store (real parts -> row).  Row b at
+// scalar b * dist.
+struct RealFullArgs {
+  int64_t n, batch, dist;
+  int use_scale;
+  double scale;
+};
+hipError_t launch_real_full(int prec, int which, void *rows, void *scr, const RealFullArgs &a,
+                            hipStream_t s);
 
 }  // namespace pfx

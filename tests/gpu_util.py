@@ -38,10 +38,11 @@ def tol(prec: str, n: int) -> float:
 # Which lengths the library must accept: every product of a radix plan in
 # pfx_plans.inc has tiled kernels (C2C, and R2C/C2R of twice the length), any
 # length up to the direct-DFT limit (2n complex values in 64 KiB of LDS) runs
-# the fallback, and a longer length that splits into two tiled lengths runs
-# the four-step plan (an even real length: its half as C2C).  A plan outside
-# these must be refused; every other plan must be created (a refusal there is
-# a failure, not a skip).
+# the fallback, a longer length that splits into two tiled lengths runs the
+# four-step plan, and any other length up to 2^23 Bluestein's chirp
+# convolution (an even real length: its half as C2C; an odd one: the whole
+# row as C2C).  A plan outside these must be refused; every other plan must
+# be created (a refusal there is a failure, not a skip).
 NAIVE_MAX = {"f32": 4096, "f64": 2048}
 
 
@@ -72,12 +73,15 @@ def long_split(n: int):
     return best
 
 
+CHIRP_MAX = 1 << 23   # Bluestein: the M >= 2n - 1 convolution must split (M <= 4096^2)
+
+
 def c2c_supported(prec: str, n: int) -> bool:
     return n in tiled_lengths() or n <= NAIVE_MAX["f64" if prec == "f64" else "f32"] or \
-        long_split(n) is not None
+        long_split(n) is not None or n <= CHIRP_MAX
 
 
 def real_supported(prec: str, n: int) -> bool:
     return (n % 2 == 0 and n // 2 in tiled_lengths()) or \
         n <= NAIVE_MAX["f64" if prec == "f64" else "f32"] or \
-        (n % 2 == 0 and c2c_supported(prec, n // 2))
+        (n % 2 == 0 and c2c_supported(prec, n // 2)) or (n % 2 == 1 and c2c_supported(prec, n))

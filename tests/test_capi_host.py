@@ -109,12 +109,11 @@ def test_batch_product_overflow_rejected():
     assert b"overflows" in L.pfx_last_error_message()
 
 
-@pytest.mark.parametrize("prec,n", [(0, 8191), (0, 4097), (1, 2 * 2053), (1, 4096 + 2),
-                                    (0, 3 * 4099)])
+@pytest.mark.parametrize("prec,n", [(0, (1 << 23) + 1), (1, (1 << 23) + 9), (0, 16777259)])
 def test_lengths_without_a_kernel_are_refused_on_the_host(prec, n):
-    """C2C lengths with no tiled plan beyond the direct-DFT limit (4096
-    complex<float>, 2048 complex<double>) that do not split into two tiled
-    lengths either are refused before any device work, with
+    """C2C lengths with no tiled plan beyond the direct-DFT limit that
+    neither split into two tiled lengths nor fit the chirp convolution
+    (n <= 2^23) are refused before any device work, with
     PFX_ERR_UNSUPPORTED; the GPU tests hold the complementary rule (every
     other length must plan)."""
     import sys
@@ -128,6 +127,7 @@ def test_lengths_without_a_kernel_are_refused_on_the_host(prec, n):
     assert h.value is None
     assert b"fallback" in L.pfx_last_error_message()
     assert b"no split into two tiled lengths" in L.pfx_last_error_message()
+    assert b"chirp" in L.pfx_last_error_message()
 
 
 def test_long_split_rule():
@@ -141,7 +141,9 @@ def test_long_split_rule():
     assert long_split(6144) == (64, 96)
     assert long_split(8191) is None and long_split(4097) is None
     assert c2c_supported("f64", 3072) and c2c_supported("f32", 1 << 20)
-    assert real_supported("f32", 32768) and not real_supported("f32", 8194)
+    assert c2c_supported("f32", 8191) and not c2c_supported("f32", (1 << 23) + 1)
+    assert real_supported("f32", 32768) and real_supported("f32", 8194)
+    assert real_supported("f64", 4097) and not real_supported("f32", 2 * ((1 << 23) + 1))
 
 
 def test_tune_cache_load_counts_dropped_lines(tmp_path):

@@ -1,5 +1,7 @@
 """Lengths beyond one kernel: the four-step C2C plan (n = n1 * n2, both
-factors tiled) and the long R2C / C2R plans built on it, against the oracle.
+factors tiled), Bluestein's chirp convolution for lengths that split into
+no tiled pair, and the long R2C / C2R plans built on them, against the
+oracle.
 
 hipFFT, which the backend replaces (backend/hipfft/fft_backend_hipfft.hpp:
 294-314), plans any length; here a length with no tiled kernel beyond the
@@ -141,6 +143,79 @@ def test_long_r2c_c2r_vs_oracle(prec, n):
     assert rel_l2(to_host(d)[:, :n], real.astype(np.float64)) <= 2 * tol(prec, n)
 
 
+CHIRP_CASES = [  # lengths that split into no tiled pair: Bluestein
+    ("f32", 8191, "row"), ("f32", 4099, "col"), ("f64", 2053, "gen"),
+    ("f32", 10007, "gen"), ("f64", 5000, "row"), ("f32", 5000, "col"),
+]
+
+
+@pytest.mark.parametrize("case", CHIRP_CASES, ids=lambda c: "-".join(map(str, c)))
+def test_chirp_c2c_vs_oracle(case):
+    """Bluestein: the chirp premultiply into zero-padded rows of M >= 2n - 1,
+    two M-point transforms around the filter's spectrum, the chirp
+    postmultiply; rows, columns and strided layouts with gaps (elements the
+    plan does not own come back bit-identical), both directions, a scale."""
+    import parafaft_amd as P
+    prec, n, layout = case
+    assert long_split(n) is None
+    batch, stride, dist = _geometry(n, layout)
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    size = (batch - 1) * dist + (n - 1) * stride + 1
+    x = O.uniform_complex((size,), seed=3 * n + len(layout)).astype(cdt)
+    owned = np.zeros(size, bool)
+    idx = (np.arange(batch)[:, None] * dist + np.arange(n)[None, :] * stride).ravel()
+    owned[idx] = True
+    plan = P.Plan.c2c(prec, n, batch, stride, dist)
+    M = 1
+    while M < 2 * n - 1:
+        M *= 2
+    assert plan.describe()[0].endswith(f"bluestein_n{n}_m{M}"), plan.describe()
+    for direction, sign, scale in ((0, -1, 1.0), (1, +1, 1.0 / n)):
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, stride, dist, sign)
+        ref *= scale
+        d = to_dev(x)
+        plan.execute(direction, d, scale=scale)
+        got = to_host(d)
+        e = rel_l2(got[owned], ref[owned])
+        assert e <= tol(prec, n), (direction, e)
+        assert np.array_equal(got[~owned].view(np.uint8), x[~owned].view(np.uint8))
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 4097), ("f64", 2049), ("f32", 8191),
+                                    ("f32", 2 * 8191), ("f64", 6001)])
+def test_odd_and_chirp_real_vs_oracle(prec, n):
+    """Odd real lengths beyond the direct-DFT limit run the whole row as a
+    C2C on scratch rows (R2C: the reals with zero imaginary parts; C2R: the
+    Hermitian extension); an even length whose half has no tiled pair runs
+    its half through the chirp convolution."""
+    import parafaft_amd as P
+    batch = 3
+    h = n // 2 + 1
+    pd = 2 * h + 2
+    rdt = np.float32 if prec == "f32" else np.float64
+    real = O.uniform(batch * n, seed=n + 7).reshape(batch, n).astype(rdt)
+    pad = np.zeros((batch, pd), dtype=rdt)
+    pad[:, :n] = real
+    ref = pad.astype(np.float64).copy()
+    O.r2c_inplace(ref, n, batch, 1, pd)
+    r2c = P.Plan.r2c(prec, n, batch, 1, pd)
+    c2r = P.Plan.c2r(prec, n, batch, 1, pd)
+    d = to_dev(pad)
+    r2c.execute(0, d)
+    got = to_host(d)
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    refc = ref[:, :2 * h].copy().view(np.complex128)
+    assert rel_l2(got[:, :2 * h].copy().view(cdt), refc) <= tol(prec, n)
+    ref2 = ref.copy()
+    O.c2r_inplace(ref2, n, batch, 1, pd)
+    d2 = to_dev(ref.astype(rdt))
+    c2r.execute(1, d2)
+    assert rel_l2(to_host(d2)[:, :n], ref2[:, :n]) <= tol(prec, n)
+    c2r.execute(1, d, scale=1.0 / n)
+    assert rel_l2(to_host(d)[:, :n], real.astype(np.float64)) <= 2 * tol(prec, n)
+
+
 @pytest.mark.parametrize("fast", [True, False])
 def test_driver_with_a_long_axis(fast):
     """A 3 x 8192 C2C and a 2 x 16384 R2C through the one-rank drivers (the
@@ -171,18 +246,19 @@ def test_driver_with_a_long_axis(fast):
 
 
 def test_long_refusals():
-    """Lengths that do not split into two tiled lengths are refused; a long
+    """Lengths beyond the chirp limit that do not split are refused; a long
     plan has no segmented or out-of-place real form and no persistent body;
     in a batch it runs on its own (direct) beside the grouped launches."""
     import ctypes as C
     import parafaft_amd as P
     from parafaft_amd import _capi as A
     from parafaft_amd.batch import Batch
+    big = (1 << 23) + 1   # splits into no tiled pair and exceeds the chirp limit
     with pytest.raises(P.PfxError) as e:
-        P.Plan.c2c("f32", 8191, 2, 1, 8191)
+        P.Plan.c2c("f32", big, 2, 1, big)
     assert e.value.status == 2
     with pytest.raises(P.PfxError):
-        P.Plan.r2c("f32", 8194, 2, 1, 2 * 4098)
+        P.Plan.r2c("f32", 2 * big, 2, 1, 2 * (big + 1))
     n = 8192
     plan = P.Plan.c2c("f32", n, 2, 1, n)
     d = to_dev(np.zeros(2 * n, np.complex64))
