@@ -57,15 +57,15 @@ const char *pfx_version(void);
  * transform b is data[b*dist + j*stride] (complex units).  Both directions
  * share one plan.
  * Lengths: one tiled kernel where the length has a radix plan (up to 4096);
- * else the direct O(n^2) DFT up to 4096 (complex<float>) / 2048
- * (complex<double>); beyond that a four-step plan when the length splits
- * into two lengths with radix plans (n = n1 * n2, e.g. 8192 = 64 * 128, up
- * to 4096^2): two tiled passes, a twiddle pass and a transpose through a
- * scratch buffer of the plan's footprint; any other length up to 2^23
- * through Bluestein's chirp convolution (zero-padded rows of M >= 2n - 1,
- * two M-point transforms).  These plans own their scratch, so executions
- * of one such plan must not overlap.  Longer lengths that do not split:
- * PFX_ERR_UNSUPPORTED.
+ * else a four-step plan when the length splits into two lengths with radix
+ * plans (n = n1 * n2, e.g. 100 = 10 * 10, 8192 = 64 * 128, up to 4096^2):
+ * two tiled passes, a twiddle pass and a transpose through a scratch buffer
+ * of the plan's footprint; any other length up to 2^23 through Bluestein's
+ * chirp convolution (zero-padded rows of M >= 2n - 1, two M-point
+ * transforms).  These plans own their scratch, so executions of one such
+ * plan must not overlap.  Longer lengths that do not split:
+ * PFX_ERR_UNSUPPORTED.  (The direct O(n^2) DFT, up to 4096 complex<float> /
+ * 2048 complex<double>, remains for pfx_execute_c2c_noalloc.)
  * Replaces: HipFFTBackend::create_stage_plan -> hipfftPlanMany(Z2Z/C2C)
  *   (backend/hipfft/fft_backend_hipfft.hpp:294-314); geometry as in
  *   FFTWBackend::create_stage_plan (backend/fftw3/fft_backend_fftw.hpp:387-424).
@@ -88,12 +88,11 @@ int pfx_plan_create_c2c_many(pfx_plan *plan, int precision, int64_t length,
  * In-place padded R2C plan: row b starts at scalar b*dist_scalars; its
  * `length` reals are replaced by length/2+1 complex values (complex row
  * distance dist_scalars/2).  stride must be 1 (the only value ParaFaFT uses).
- * Lengths: even lengths whose half has a radix plan run one kernel; others
- * up to the direct-DFT limit the fallback; longer even lengths the
- * half-length C2C plan (as above) of the rows read as complex pairs, with a
- * post-pass (C2R: a pre-pass); longer odd lengths a length-n C2C plan on
- * scratch rows (the reals with zero imaginary parts; C2R: the Hermitian
- * extension).
+ * Lengths: even lengths whose half has a radix plan run one kernel; other
+ * even lengths the half-length C2C plan (as above) of the rows read as
+ * complex pairs, with a post-pass (C2R: a pre-pass); odd lengths a length-n
+ * C2C plan on scratch rows (the reals with zero imaginary parts; C2R: the
+ * Hermitian extension).
  * Replaces: HipFFTBackend::create_r2c_inplace_plan (fft_backend_hipfft.hpp:361-384),
  *   FFTWBackend::create_r2c_inplace_plan (fft_backend_fftw.hpp:439-469).
  */
@@ -174,12 +173,14 @@ int pfx_execute_segmented(pfx_plan plan, int direction, void *data, const pfx_se
 int pfx_plan_destroy(pfx_plan plan);
 
 /*
- * One C2C transform with no plan and no allocation: the plan
- * pfx_plan_create_c2c(precision, length, batch, stride, dist) would build
- * (its default kernel, not a tuned one) is set up on the stack, its twiddle
- * tables are generated into a fixed device scratch array of the library (a
- * kernel, on `stream`), and the transform runs as pfx_execute_scaled would:
- * same kernel, same tables, so the same bits.  Nothing is allocated on the
+ * One C2C transform with no plan and no allocation: for a length with a
+ * tiled kernel, the plan pfx_plan_create_c2c(precision, length, batch,
+ * stride, dist) would build (its default kernel, not a tuned one) is set up
+ * on the stack, its twiddle tables are generated into a fixed device scratch
+ * array of the library (a kernel, on `stream`), and the transform runs as
+ * pfx_execute_scaled would: same kernel, same tables, so the same bits.  A
+ * length without a tiled kernel runs the direct O(n^2) DFT (the planned
+ * path's four-step and chirp plans need scratch of their own).  Nothing is allocated on the
  * device or the host and no call synchronises, so callers that cannot call
  * hipMalloc (code under HIP graph capture, tight serving loops) can use it.
  * Limits: PFX_F32 / PFX_F64, length <= 4096 (larger lengths are refused with

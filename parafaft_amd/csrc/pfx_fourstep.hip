@@ -81,16 +81,19 @@ __global__ void __launch_bounds__(256) fourstep_transpose_rows(const V2<T>* src,
   }
 }
 
-// Step 4, strided transforms: one element per thread, the batch index
-// fastest (contiguous when dist == 1).
+// Step 4, strided transforms and short rows: one element per thread, the
+// batch index fastest when dist == 1 (contiguous), else the output element
+// fastest (contiguous for rows; a short row's gather stays inside it).
 template <typename T>
 __global__ void __launch_bounds__(256) fourstep_transpose_strided(const V2<T>* src, V2<T>* dst,
                                                                   FourStepArgs a) {
   const int64_t n = a.n1 * a.n2, total = n * a.batch;
   const T sc = T(a.scale);
+  const bool bfast = a.dist == 1;
   for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
        t += (int64_t)gridDim.x * blockDim.x) {
-    const int64_t b = t % a.batch, o = t / a.batch;  // output element o = n2 k1 + k2
+    const int64_t b = bfast ? t % a.batch : t / n;
+    const int64_t o = bfast ? t / a.batch : t - b * n;  // output element o = n2 k1 + k2
     const int64_t k1 = o / a.n2, k2 = o - k1 * a.n2;
     V2<T> v = src[b * a.dist + (k1 + a.n1 * k2) * a.stride];
     if (a.use_scale) v = scl<T>(v, sc);
@@ -323,7 +326,7 @@ hipError_t launch_fourstep_twiddle(int prec, void* buf, const void* tw, const Fo
 
 hipError_t launch_fourstep_transpose(int prec, const void* src, void* dst, const FourStepArgs& a,
                                      hipStream_t s) {
-  if (a.stride == 1) {
+  if (a.stride == 1 && a.n1 >= 32 && a.n2 >= 32) {
     const int64_t tiles = ((a.n1 + 31) / 32) * ((a.n2 + 31) / 32) * a.batch;
     const int g = (int)(tiles < 65536 ? tiles : 65536);
     if (prec)

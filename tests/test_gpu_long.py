@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_util import long_split, rel_l2, to_dev, to_host, tol
+from tests.gpu_util import long_split, rel_l2, tiled_lengths, to_dev, to_host, tol
 
 pytestmark = pytest.mark.gpu
 
@@ -279,3 +279,54 @@ def test_long_refusals():
     assert rel_l2(got[n:].reshape(64, 64), O.fftn(x1.astype(np.complex128))) <= tol("f32", 4096)
     with pytest.raises(P.PfxError):
         b.execute_persistent(0)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 7), ("f32", 100), ("f32", 4093), ("f64", 2047),
+                                    ("f64", 99)])
+def test_direct_dft_kept_under_the_knob(prec, n, monkeypatch):
+    """PFX_LONG_MIN (a measurement knob) keeps the direct O(n^2) DFT below
+    its value, as the allocation-free entry point always does: C2C and
+    R2C / C2R against the oracle, so the fallback kernel stays covered."""
+    import parafaft_amd as P
+    monkeypatch.setenv("PFX_LONG_MIN", str(1 << 30))
+    batch = 3
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    x = O.uniform_complex((batch * n,), seed=n + 11).astype(cdt)
+    plan = P.Plan.c2c(prec, n, batch, 1, n)
+    assert "direct_dft" in plan.describe()[0], plan.describe()
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, 1, n, -1)
+    d = to_dev(x)
+    plan.execute(0, d)
+    assert rel_l2(to_host(d), ref) <= tol(prec, n)
+    h = n // 2 + 1
+    pd = 2 * h
+    rdt = np.float32 if prec == "f32" else np.float64
+    real = O.uniform(batch * n, seed=n + 12).reshape(batch, n).astype(rdt)
+    pad = np.zeros((batch, pd), rdt)
+    pad[:, :n] = real
+    refr = pad.astype(np.float64).copy()
+    O.r2c_inplace(refr, n, batch, 1, pd)
+    r2c = P.Plan.r2c(prec, n, batch, 1, pd)
+    if n % 2 == 0 and n // 2 in tiled_lengths():
+        return   # a tiled R2C kernel exists
+    assert "direct_dft" in r2c.describe()[0], r2c.describe()
+    d = to_dev(pad)
+    r2c.execute(0, d)
+    got = to_host(d).copy().view(cdt)
+    assert rel_l2(got, refr.view(np.complex128)) <= tol(prec, n)
+    c2r = P.Plan.c2r(prec, n, batch, 1, pd)
+    c2r.execute(1, d, scale=1.0 / n)
+    assert rel_l2(to_host(d)[:, :n], real.astype(np.float64)) <= 2 * tol(prec, n)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 7), ("f32", 100), ("f64", 99), ("f32", 1000)])
+def test_short_lengths_without_a_kernel_run_long_plans(prec, n):
+    """By default a length with no tiled kernel runs a long plan (four-step
+    when it splits into two tiled lengths, else the chirp convolution), not
+    the direct DFT."""
+    import parafaft_amd as P
+    plan = P.Plan.c2c(prec, n, 2, 1, n)
+    name = plan.describe()[0]
+    assert ("fourstep" in name) == (long_split(n) is not None), name
+    assert "direct_dft" not in name

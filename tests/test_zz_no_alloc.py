@@ -46,6 +46,23 @@ def test_noalloc_matches_planned_transform_twice():
         assert rel_l2(got[idx], ref[idx]) <= tol(prec, n), (prec, n)
 
 
+def test_noalloc_direct_dft_for_lengths_without_a_kernel():
+    """A length with no tiled kernel runs the direct DFT here (the planned
+    path takes a long plan, which needs scratch): against the oracle."""
+    from parafaft_amd import _capi as A
+    L = A.load()
+    for prec, n, batch in (("f32", 100, 3), ("f64", 7, 2)):
+        cdt = np.complex64 if prec == "f32" else np.complex128
+        x = O.uniform_complex((batch * n,), seed=n).astype(cdt)
+        d = to_dev(x)
+        rc = L.pfx_execute_c2c_noalloc(A.PFX_F32 if prec == "f32" else A.PFX_F64, n, batch, 1, n,
+                                       0, C.c_void_p(d.data_ptr()), 1.0, None)
+        A.check(rc, "pfx_execute_c2c_noalloc")
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, 1, n, -1)
+        assert rel_l2(to_host(d), ref) <= tol(prec, n), (prec, n)
+
+
 def test_noalloc_refuses_lengths_past_the_scratch():
     from parafaft_amd import _capi as A
     L = A.load()
