@@ -176,8 +176,8 @@ template <typename T> static void test_different_pointers() {
 }
 
 // Test 5: R2C / C2R in place on padded rows
-template <typename T> static void test_r2c_c2r() {
-  const int N = 64, batch = 5, h = N / 2 + 1, pd = 2 * h;
+template <typename T> static void test_r2c_c2r(const int N = 64) {
+  const int batch = 5, h = N / 2 + 1, pd = 2 * h;
   std::vector<double> u(N * batch);
   orc_uniform(u.data(), u.size(), 5);
   std::vector<T> pad(pd * batch, T(0));
@@ -210,6 +210,36 @@ template <typename T> static void test_r2c_c2r() {
   for (int b = 0; b < batch; ++b)
     for (int i = 0; i < N; ++i) e = std::max(e, std::abs(double(out[b * pd + i]) / N - double(pad[b * pd + i])));
   check(e < tol<T>(), "R2C -> C2R roundtrip", e);
+}
+
+// Test 5b: lengths without a single-kernel plan go through create_stage_plan
+// like any other (hipfftPlanMany accepts any length): the four-step plan for
+// lengths that split into two tiled lengths, the chirp convolution else.
+template <typename T> static void test_any_length(const int N) {
+  const int batch = 3;
+  std::vector<double> u(2 * N * batch);
+  orc_uniform(u.data(), u.size(), 7 + N);
+  cvec<T> h(N * batch);
+  for (int i = 0; i < N * batch; ++i) h[i] = std::complex<T>(T(u[2 * i]), T(u[2 * i + 1]));
+  std::vector<std::complex<double>> orig(h.begin(), h.end());
+  parafaft::MI355Backend<T> be(1);
+  typename parafaft::MI355Backend<T>::ComplexBuffer d(N * batch);
+  h2d(d.data(), h);
+  be.create_stage_plan(0, N, batch, d.data(), 1, N);
+  be.execute_stage(0, FFTDirection::Forward, d.data());
+  be.sync();
+  cvec<T> f(N * batch);
+  d2h(f, d.data());
+  std::vector<std::complex<double>> ref(orig);
+  orc_c2c_batched(reinterpret_cast<double *>(ref.data()), N, batch, 1, N, -1, 1);
+  char what[64];
+  std::snprintf(what, sizeof(what), "length %d forward vs oracle", N);
+  check(max_err(f, ref) < tol<T>() * N, what, max_err(f, ref));
+  be.execute_stage(0, FFTDirection::Backward, d.data());
+  be.sync();
+  d2h(h, d.data());
+  std::snprintf(what, sizeof(what), "length %d roundtrip", N);
+  check(max_err(h, orig, 1.0 / N) < tol<T>(), what, max_err(h, orig, 1.0 / N));
 }
 
 // Test 6: errors surface as std::runtime_error (check_hip style)
@@ -253,6 +283,13 @@ int main() {
   test_multiple_stages<float>();
   test_different_pointers<float>();
   test_r2c_c2r<float>();
+  std::printf("lengths without a single-kernel plan\n");
+  for (int n : {100, 1000, 8191, 8192}) {
+    test_any_length<float>(n);
+    test_any_length<double>(n);
+  }
+  test_r2c_c2r<float>(1001);
+  test_r2c_c2r<double>(2 * 1009);
   test_measure_planning();
   test_errors();
   std::printf("%d failure(s)\n", failures);
