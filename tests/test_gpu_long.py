@@ -330,3 +330,40 @@ def test_short_lengths_without_a_kernel_run_long_plans(prec, n):
     name = plan.describe()[0]
     assert ("fourstep" in name) == (long_split(n) is not None), name
     assert "direct_dft" not in name
+
+
+def _random_lengths(seed, count):
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        n = int(rng.integers(5, 12000))
+        prec = "f64" if rng.random() < 0.4 else "f32"
+        layout = ("row", "col", "gen")[int(rng.integers(0, 3))]
+        out.append((i, prec, n, layout))
+    return out
+
+
+@pytest.mark.parametrize("case", _random_lengths(20261204, 24),
+                         ids=lambda c: "-".join(map(str, c)))
+def test_random_lengths_vs_oracle(case):
+    """Seeded random lengths 5..12000 (tiled, four-step or chirp, whichever
+    the length takes), random layout and precision: forward against the
+    oracle, elements the plan does not own untouched."""
+    import parafaft_amd as P
+    i, prec, n, layout = case
+    batch, stride, dist = _geometry(n, layout)
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    size = (batch - 1) * dist + (n - 1) * stride + 1
+    x = O.uniform_complex((size,), seed=9000 + i).astype(cdt)
+    owned = np.zeros(size, bool)
+    idx = (np.arange(batch)[:, None] * dist + np.arange(n)[None, :] * stride).ravel()
+    owned[idx] = True
+    plan = P.Plan.c2c(prec, n, batch, stride, dist)
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, stride, dist, -1)
+    d = to_dev(x)
+    plan.execute(0, d)
+    got = to_host(d)
+    assert rel_l2(got[owned], ref[owned]) <= tol(prec, n), (plan.describe(), rel_l2(got[owned],
+                                                                                  ref[owned]))
+    assert np.array_equal(got[~owned].view(np.uint8), x[~owned].view(np.uint8))
