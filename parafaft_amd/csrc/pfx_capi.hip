@@ -156,12 +156,17 @@ int base_prec(int precision) {
 //  LONG_REAL_FULL (odd real length): `c` is a length-n C2C plan on scratch
 //    rows of n complex values, the real rows loaded into / stored from them.
 enum { LONG_FOURSTEP = 0, LONG_CHIRP = 1, LONG_REAL_HALF = 2, LONG_REAL_FULL = 3 };
+// Scratch rows of the chirp and odd-real modes cover at most this many bytes
+// (the batch runs in slices of that many rows): bounded memory, and a slice
+// stays in the Infinity Cache between its passes.
+constexpr int64_t kLongRowsBytes = int64_t(256) << 20;
 constexpr int64_t kChirpMaxLength = int64_t(1) << 23;  // M <= 2^24 = 4096^2 splits
 struct pfx_plan_t;
 struct LongPlan {
   int mode = LONG_FOURSTEP;
   int64_t n1 = 0, n2 = 0;
   int64_t M = 0;            // LONG_CHIRP: convolution length
+  int64_t rows = 0;         // LONG_CHIRP / LONG_REAL_FULL: scratch rows (a slice of the batch)
   pfx_plan_t *a = nullptr;
   pfx_plan_t *c = nullptr;
   void *scratch = nullptr;  // the mode's work rows (four-step: one outer
@@ -345,10 +350,11 @@ static int make_long(pfx_plan p) {
       lp->mode = LONG_CHIRP;
       lp->M = 1;
       while (lp->M < 2 * n - 1) lp->M <<= 1;
-      rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->M, total, 1, lp->M, 1, 0);
+      lp->rows = std::max<int64_t>(1, std::min<int64_t>(total, kLongRowsBytes / (lp->M * (int64_t)esz)));
+      rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->M, lp->rows, 1, lp->M, 1, 0);
       if (rc != PFX_SUCCESS) return rc;
       if (total > 0)
-        PFX_HIP(hipMalloc(&lp->scratch, (size_t)(total * lp->M) * esz), "hipMalloc chirp rows");
+        PFX_HIP(hipMalloc(&lp->scratch, (size_t)(lp->rows * lp->M) * esz), "hipMalloc chirp rows");
       PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc chirp");
       PFX_HIP(hipMalloc(&lp->filt, (size_t)(2 * lp->M) * esz), "hipMalloc chirp filters");
       PFX_HIP(make_chirp(bp, lp->tw, n, nullptr), "chirp");
@@ -372,10 +378,11 @@ static int make_long(pfx_plan p) {
     PFX_HIP(make_twiddles(bp, n, lp->tw, nullptr), "long-plan twiddles");
   } else {
     lp->mode = LONG_REAL_FULL;
-    rc = plan_create(&lp->c, p->precision, PFX_C2C, n, total, 1, n, 1, 0);
+    lp->rows = std::max<int64_t>(1, std::min<int64_t>(total, kLongRowsBytes / (n * (int64_t)esz)));
+    rc = plan_create(&lp->c, p->precision, PFX_C2C, n, lp->rows, 1, n, 1, 0);
     if (rc != PFX_SUCCESS) return rc;
     if (total > 0)
-      PFX_HIP(hipMalloc(&lp->scratch, (size_t)(total * n) * esz), "hipMalloc real rows");
+      PFX_HIP(hipMalloc(&lp->scratch, (size_t)(lp->rows * n) * esz), "hipMalloc real rows");
   }
   PFX_HIP(hipStreamSynchronize(nullptr), "long-plan tables");
   return PFX_SUCCESS;
@@ -580,39 +587,49 @@ static int run_long(pfx_plan p, int direction, void *data, double scale, hipStre
   LongPlan *lp = p->lp;
   const int bp = base_prec(p->precision);
   const size_t esz = bp ? 16 : 8;
+  // The chirp and odd-real modes run the batch in slices of lp->rows
+  // transforms; the sub-plan always transforms all scratch rows (on the
+  // last, short slice the rows past it are stale and unused).
   if (lp->mode == LONG_CHIRP) {
+    const int64_t total = p->inner * p->outer;
     ChirpArgs ca;
     ca.n = p->length;
     ca.M = lp->M;
-    ca.total = p->inner * p->outer;
     ca.inner = p->inner;
     ca.dist = p->dist;
     ca.dist_outer = p->dist_outer;
     ca.stride = p->stride;
     ca.backward = direction == PFX_BACKWARD;
     ca.scale = scale;
-    PFX_HIP(launch_chirp_pre(bp, data, lp->scratch, lp->tw, ca, s), "chirp premultiply");
-    int rc = pfx_execute_scaled(lp->c, PFX_FORWARD, lp->scratch, 1.0, s);
-    if (rc != PFX_SUCCESS) return rc;
     const char *B = static_cast<const char *>(lp->filt) + (ca.backward ? (size_t)lp->M * esz : 0);
-    PFX_HIP(launch_chirp_mul(bp, lp->scratch, B, ca, s), "chirp filter");
-    rc = pfx_execute_scaled(lp->c, PFX_BACKWARD, lp->scratch, 1.0, s);
-    if (rc != PFX_SUCCESS) return rc;
-    PFX_HIP(launch_chirp_post(bp, lp->scratch, data, lp->tw, ca, s), "chirp postmultiply");
+    for (int64_t t0 = 0; t0 < total; t0 += lp->rows) {
+      ca.t0 = t0;
+      ca.total = std::min(lp->rows, total - t0);
+      PFX_HIP(launch_chirp_pre(bp, data, lp->scratch, lp->tw, ca, s), "chirp premultiply");
+      int rc = pfx_execute_scaled(lp->c, PFX_FORWARD, lp->scratch, 1.0, s);
+      if (rc != PFX_SUCCESS) return rc;
+      PFX_HIP(launch_chirp_mul(bp, lp->scratch, B, ca, s), "chirp filter");
+      rc = pfx_execute_scaled(lp->c, PFX_BACKWARD, lp->scratch, 1.0, s);
+      if (rc != PFX_SUCCESS) return rc;
+      PFX_HIP(launch_chirp_post(bp, lp->scratch, data, lp->tw, ca, s), "chirp postmultiply");
+    }
     return PFX_SUCCESS;
   }
   if (lp->mode == LONG_REAL_FULL) {
     RealFullArgs fa;
     fa.n = p->length;
-    fa.batch = p->inner;
     fa.dist = p->dist;
     fa.use_scale = scale != 1.0;
     fa.scale = scale;
     const bool r2c = p->kind == PFX_R2C;
-    PFX_HIP(launch_real_full(bp, r2c ? 0 : 2, data, lp->scratch, fa, s), "real rows load");
-    int rc = pfx_execute_scaled(lp->c, r2c ? PFX_FORWARD : PFX_BACKWARD, lp->scratch, 1.0, s);
-    if (rc != PFX_SUCCESS) return rc;
-    PFX_HIP(launch_real_full(bp, r2c ? 1 : 3, data, lp->scratch, fa, s), "real rows store");
+    for (int64_t b0 = 0; b0 < p->inner; b0 += lp->rows) {
+      fa.batch = std::min(lp->rows, p->inner - b0);
+      char *rows = static_cast<char *>(data) + (size_t)(b0 * p->dist) * (esz / 2);
+      PFX_HIP(launch_real_full(bp, r2c ? 0 : 2, rows, lp->scratch, fa, s), "real rows load");
+      int rc = pfx_execute_scaled(lp->c, r2c ? PFX_FORWARD : PFX_BACKWARD, lp->scratch, 1.0, s);
+      if (rc != PFX_SUCCESS) return rc;
+      PFX_HIP(launch_real_full(bp, r2c ? 1 : 3, rows, lp->scratch, fa, s), "real rows store");
+    }
     return PFX_SUCCESS;
   }
   if (p->kind == PFX_C2C) {

@@ -209,7 +209,7 @@ __device__ __forceinline__ int64_t xform_base(int64_t t, const ChirpArgs& a) {
   return o * a.dist_outer + i * a.dist;
 }
 
-// scratch row t = (x_t[j] w'[j] for j < n, then zeros), M per row
+// scratch row t = (x_(t0+t)[j] w'[j] for j < n, then zeros), M per row
 template <typename T>
 __global__ void __launch_bounds__(256) chirp_pre_kernel(const V2<T>* x, V2<T>* scr,
                                                         const V2<T>* w, ChirpArgs a) {
@@ -219,7 +219,7 @@ __global__ void __launch_bounds__(256) chirp_pre_kernel(const V2<T>* x, V2<T>* s
     const int64_t t = e / a.M, j = e - t * a.M;
     V2<T> v = mk<T>(T(0), T(0));
     if (j < a.n) {
-      const V2<T> xv = x[xform_base(t, a) + j * a.stride];
+      const V2<T> xv = x[xform_base(a.t0 + t, a) + j * a.stride];
       v = a.backward ? cmulc<T>(xv, w[j]) : cmul<T>(xv, w[j]);
     }
     scr[e] = v;
@@ -235,7 +235,7 @@ __global__ void __launch_bounds__(256) chirp_mul_kernel(V2<T>* scr, const V2<T>*
     scr[e] = cmul<T>(scr[e], B[e % a.M]);
 }
 
-// x_t[k] = scale / M * w'[k] * scratch row t [k], k < n
+// x_(t0+t)[k] = scale / M * w'[k] * scratch row t [k], k < n
 template <typename T>
 __global__ void __launch_bounds__(256) chirp_post_kernel(const V2<T>* scr, V2<T>* x,
                                                          const V2<T>* w, ChirpArgs a) {
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(256) chirp_post_kernel(const V2<T>* scr, V2<T>
     const int64_t t = e / a.n, k = e - t * a.n;
     const V2<T> s = scr[t * a.M + k];
     const V2<T> v = a.backward ? cmulc<T>(s, w[k]) : cmul<T>(s, w[k]);
-    x[xform_base(t, a) + k * a.stride] = scl<T>(v, sc);
+    x[xform_base(a.t0 + t, a) + k * a.stride] = scl<T>(v, sc);
   }
 }
 

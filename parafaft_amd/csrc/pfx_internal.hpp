@@ -156,11 +156,11 @@ hipError_t launch_real_post(int prec, void *data, const void *tw, const RealPass
                             hipStream_t s);
 hipError_t launch_real_pre(int prec, void *data, const void *tw, const RealPassArgs &a,
                            hipStream_t s);
-// Bluestein (pfx_fourstep.hip): transform t of `total` at
-// (t / inner) dist_outer + (t % inner) dist, element j at j * stride;
-// scratch rows of M >= 2n - 1.
+// Bluestein (pfx_fourstep.hip): `total` transforms from t0 on, transform t
+// at (t / inner) dist_outer + (t % inner) dist, element j at j * stride;
+// scratch rows of M >= 2n - 1 (row 0 for transform t0).
 struct ChirpArgs {
-  int64_t n, M, total, inner, dist, dist_outer, stride;
+  int64_t n, M, t0, total, inner, dist, dist_outer, stride;
   int backward;
   double scale;
 };

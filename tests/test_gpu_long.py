@@ -367,3 +367,36 @@ def test_random_lengths_vs_oracle(case):
     assert rel_l2(got[owned], ref[owned]) <= tol(prec, n), (plan.describe(), rel_l2(got[owned],
                                                                                   ref[owned]))
     assert np.array_equal(got[~owned].view(np.uint8), x[~owned].view(np.uint8))
+
+
+def test_chirp_and_odd_real_batches_run_in_slices():
+    """Batches whose chirp (odd-real) scratch rows exceed 256 MB run in
+    slices of rows through one bounded scratch: 20000 transforms of 1000
+    points in columns (M = 2048: two slices, the second short) and 20000
+    odd real rows of 1001 (FP64), against the oracle."""
+    import parafaft_amd as P
+    n, batch = 1000, 20000
+    x = O.uniform_complex((n * batch,), seed=31).astype(np.complex64)
+    plan = P.Plan.c2c("f32", n, batch, batch, 1)
+    assert "bluestein" in plan.describe()[0]
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, batch, batch, 1, -1)
+    d = to_dev(x)
+    plan.execute(0, d)
+    assert rel_l2(to_host(d), ref) <= tol("f32", n)
+    del ref, d
+    n, batch = 1001, 20000
+    h = n // 2 + 1
+    pd = 2 * h
+    real = O.uniform(batch * n, seed=32).reshape(batch, n)
+    pad = np.zeros((batch, pd))
+    pad[:, :n] = real
+    refr = pad.copy()
+    O.r2c_inplace(refr, n, batch, 1, pd)
+    r2c = P.Plan.r2c("f64", n, batch, 1, pd)
+    c2r = P.Plan.c2r("f64", n, batch, 1, pd)
+    d = to_dev(pad)
+    r2c.execute(0, d)
+    assert rel_l2(to_host(d).copy().view(np.complex128), refr.view(np.complex128)) <= tol("f64", n)
+    c2r.execute(1, d, scale=1.0 / n)
+    assert rel_l2(to_host(d)[:, :n], real) <= 2 * tol("f64", n)
