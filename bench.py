@@ -806,22 +806,23 @@ def main():
             log(f"[bench] {mode}: {sms:.3f} ms/step ({ms_per_step / sms:.3f}x plain)")
             L.pfx_ndplan_destroy(nds.value)
         # What caps the split forms, from the parts measured above: both
-        # forms move the same bytes; the split column kernels hold 16
-        # columns per tile (128-byte runs; 32 columns of split operands
-        # exceed a tile's registers) against plain's 32 (256-byte runs), and
-        # the split/scale work around the matrix-core products adds VALU
-        # time to every pass (most for BF16's six products).
+        # forms move the same bytes.  FP16's column passes run radix-32
+        # sub-DFTs on the 32x32 matrix cores over 32-column tiles (256-byte
+        # runs, as plain); BF16's three-term operands do not fit those
+        # registers, so its column kernels keep 16 columns (128-byte runs).
+        # The split/scale work around the matrix-core products adds VALU
+        # and matrix-core time to every pass (most for BF16's six products).
         plain_parts = out["part_ms"]
         split["ceiling"] = {
             "plain_part_ms": plain_parts,
             "split_minus_plain_ms": {m: {k: round(v["part_ms"].get(k, 0.0) - t, 4)
                                          for k, t in plain_parts.items()}
                                      for m, v in split.items()},
-            "note": ("same bytes, both HBM-bound: axis 0 costs the split form its 16-column "
-                     "(128-byte-run) column tiles against plain's 32-column (256-byte) ones, "
-                     "and axes 2+1 the VALU work of splitting and scaling around the "
-                     "matrix-core products; plain FP32 sits at the measured HBM touch "
-                     "ceiling, so no arithmetic form can pass it")}
+            "note": ("same bytes, both HBM-bound: FP16's column passes run radix-32 "
+                     "matrix-core sub-DFTs over 32-column (256-byte-run) tiles like plain, "
+                     "BF16's keep 16-column (128-byte) tiles; every pass adds the split, "
+                     "scale and matrix-core work around the products; plain FP32 sits at "
+                     "the measured HBM touch ceiling, so no arithmetic form can pass it")}
         out["split_vs_plain"] = split
     # the other measured cases on the same run (the driver records only this
     # line): 512^3 FP64/FP32, one rank's R2C pencil share of 2048^3 on 4x4,

@@ -61,6 +61,7 @@ struct Registry {
                         {table_f64_real, group_f64_real, seg_f64_real},
                         {table_f32_w3232, group_f32_w3232, seg_f32_w3232},
                         {table_split, nullptr, nullptr},
+                        {table_split32, nullptr, nullptr},
                         {table_nt, group_nt, nullptr},
                         {table_f32_w3232nt, nullptr, nullptr}};
     for (const Src &src : srcs) {

@@ -79,6 +79,7 @@ const KernelEntry *table_f64_row(int *count);
 const KernelEntry *table_f64_col(int *count);
 const KernelEntry *table_f64_real(int *count);
 const KernelEntry *table_split(int *count);
+const KernelEntry *table_split32(int *count);
 const KernelEntry *table_f32_w3232(int *count);
 const KernelEntry *table_nt(int *count);
 const GroupLaunchFn *group_nt(int *count);
