@@ -72,12 +72,17 @@ template <int MODE> struct Split32Lds {
 };
 
 // Fill the F operand table: one (K half, matrix, lane) vector of 8 entries
-// per thread for the first 256 threads.
+// per task, 256 tasks over the workgroup's NT threads.
 template <int MODE>
+__device__ __forceinline__ void split32_fill_task(u32x4_t* F, int task);
+template <int MODE, int NT>
 __device__ __forceinline__ void split32_fill_f(u32x4_t* F) {
+#pragma unroll
+  for (int task = threadIdx.x; task < 256; task += NT) split32_fill_task<MODE>(F, task);
+}
+template <int MODE>
+__device__ __forceinline__ void split32_fill_task(u32x4_t* F, int task) {
   using S = Split32Lds<MODE>;
-  const int task = threadIdx.x;
-  if (task >= 256) return;
   const int h = task >> 7, m = (task >> 6) & 1, L = task & 63;
   const int i = L & 31, k0 = 16 * h + 8 * (L >> 5);
   if constexpr (MODE == SPLIT_F16) {
@@ -272,7 +277,7 @@ __device__ __forceinline__ void split32_col_body(const StockhamArgs& a, unsigned
 #pragma unroll
   for (int g = 0; g < PD; ++g) load_group(g);
   // tables, while the loads are in flight
-  split32_fill_f<MODE>(F);
+  split32_fill_f<MODE, 64 * W>(F);
   {
     const V2<float>* tw = static_cast<const V2<float>*>(a.ptw) + 31;
     for (int e = threadIdx.x; e < S::ntw; e += 64 * W) stw[e] = tw[e];

@@ -111,7 +111,8 @@ def test_split32_unbiased():
 
 def test_split32_is_a_tuning_candidate_and_bf16_is_not():
     """The Measure tuner sees the radix-32 kernels for FP16 column plans; the
-    BF16 form has none (its operand splits do not fit the registers)."""
+    BF16 form has none for columns (its operand splits do not fit the
+    registers of the 32-column tile)."""
     import parafaft_amd as P
     p16 = P.Plan.c2c("f32_split_f16", N, 64, 64, 1)
     _split32_candidates(p16)
