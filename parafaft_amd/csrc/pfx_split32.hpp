@@ -42,18 +42,16 @@ constexpr double C5 = 0.55557023301960222474283081394853288;  // cos(5 pi / 16)
 constexpr double C7 = 0.19509032201612826784828486847702224;  // cos(7 pi / 16)
 }  // namespace k32
 
-// cos(2 pi t / 32) by selects (no indexed table, which would live in scratch
-// memory)
-__device__ __forceinline__ float cos32(int t) {
-  int a = t & 31;
-  if (a > 16) a = 32 - a;  // cos(2 pi (32 - a) / 32) = cos(2 pi a / 32)
-  const bool neg = a > 8;
-  if (neg) a = 16 - a;     // cos(pi - x) = -cos x
-  const float c = a == 0 ? 1.f : a == 1 ? float(k32::C1) : a == 2 ? float(k::COS_PI8)
-                : a == 3 ? float(k32::C3) : a == 4 ? float(k::SQRT1_2) : a == 5 ? float(k32::C5)
-                : a == 6 ? float(k::SIN_PI8) : a == 7 ? float(k32::C7) : 0.f;
-  return neg ? -c : c;
-}
+// cos(2 pi t / 32), t < 32, as one indexed load per entry from a constant
+// table (a select chain compiles to a branching search here)
+static __constant__ float kCos32[32] = {
+    1.f, float(k32::C1), float(k::COS_PI8), float(k32::C3), float(k::SQRT1_2), float(k32::C5),
+    float(k::SIN_PI8), float(k32::C7), 0.f, -float(k32::C7), -float(k::SIN_PI8), -float(k32::C5),
+    -float(k::SQRT1_2), -float(k32::C3), -float(k::COS_PI8), -float(k32::C1), -1.f,
+    -float(k32::C1), -float(k::COS_PI8), -float(k32::C3), -float(k::SQRT1_2), -float(k32::C5),
+    -float(k::SIN_PI8), -float(k32::C7), 0.f, float(k32::C7), float(k::SIN_PI8), float(k32::C5),
+    float(k::SQRT1_2), float(k32::C3), float(k::COS_PI8), float(k32::C1)};
+__device__ __forceinline__ float cos32(int t) { return kCos32[t & 31]; }
 
 // LDS block of the kernel: F operand splits, pass-2 twiddles, exchange buffer
 // (small tables first, so their addresses fit ds_read's 16-bit offset).
@@ -131,10 +129,11 @@ template <> struct MfmaDft32<SPLIT_F16> {
     return __builtin_bit_cast(half8_t, F[S::op(h, m, t) * 64 + lane]);
   }
   // x[8 h + r]: input n = 16 h + 8 (lane / 32) + r; y[r]: output
-  // k = 8 (r / 4) + 4 (lane / 32) + r % 4.  Every lane of the wave calls this
-  // together.  The butterfly's 32 inputs sit in lanes l and l ^ 32.
+  // k = 8 (r / 4) + 4 (lane / 32) + r % 4, times `post`.  Every lane of the
+  // wave calls this together.  The butterfly's 32 inputs sit in lanes l and
+  // l ^ 32.
   __device__ static __forceinline__ void apply(const V2<float>* x, V2<float>* y, bool neg,
-                                               const u32x4_t* F) {
+                                               const u32x4_t* F, float post = 1.f) {
     const int lane = threadIdx.x & 63;
     float amax = 0.f;
 #pragma unroll
@@ -145,7 +144,9 @@ template <> struct MfmaDft32<SPLIT_F16> {
     const int sh = ex - 15 < -100 ? -100 : ex - 15;
     const uint32_t sbit = neg ? 0x80000000u : 0u;
     const float sin_ = __uint_as_float(((uint32_t)(127 - sh) << 23) | sbit);
-    const float sout = __uint_as_float(((uint32_t)(127 + sh) << 23) | sbit);
+    // a power of two times `post`: the same rounding as scaling by the power
+    // of two and then by `post`
+    const float sout = __uint_as_float(((uint32_t)(127 + sh) << 23) | sbit) * post;
     float16_t zr = {}, zi = {};
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -199,7 +200,7 @@ template <> struct MfmaDft32<SPLIT_BF16> {
     return z;
   }
   __device__ static __forceinline__ void apply(const V2<float>* x, V2<float>* y, bool neg,
-                                               const u32x4_t* F) {
+                                               const u32x4_t* F, float post = 1.f) {
     const int lane = threadIdx.x & 63;
     const uint32_t sbit = neg ? 0x80000000u : 0u;
     float16_t zr = {}, zi = {};
@@ -232,8 +233,8 @@ template <> struct MfmaDft32<SPLIT_BF16> {
     }
 #pragma unroll
     for (int r = 0; r < 16; ++r)
-      y[r] = mk<float>(__uint_as_float(__float_as_uint(zr[r]) ^ sbit),
-                       __uint_as_float(__float_as_uint(zi[r]) ^ sbit));
+      y[r] = mk<float>(__uint_as_float(__float_as_uint(zr[r]) ^ sbit) * post,
+                       __uint_as_float(__float_as_uint(zi[r]) ^ sbit) * post);
   }
 };
 
@@ -326,15 +327,13 @@ __device__ __forceinline__ void split32_col_body(const StockhamArgs& a, unsigned
       for (int h = 0; h < 2; ++h)
 #pragma unroll
         for (int r = 0; r < 8; ++r) x[8 * h + r] = buf[((k2l * 32) + 16 * h + 8 * hh + r) * 32 + c];
-      MfmaDft32<MODE>::apply(x, y, split_sign(b, k2), F);
+      MfmaDft32<MODE>::apply(x, y, split_sign(b, k2), F, sc);
       if (valid) {
         V2<float>* sp = dst + (int64_t)(k2 + 128 * hh) * st;
         const int64_t s32 = 32 * st, s256 = 256 * st;
 #pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const V2<float> o = scl<float>(y[r], sc);
-          split_st<NT>(sp + ((r >> 2) * s256 + (r & 3) * s32), SW ? swp<float>(o) : o);
-        }
+        for (int r = 0; r < 16; ++r)
+          split_st<NT>(sp + ((r >> 2) * s256 + (r & 3) * s32), SW ? swp<float>(y[r]) : y[r]);
       }
     }
   }
