@@ -25,6 +25,11 @@
 // sign with it, and cancels over any aligned power-of-two range of either
 // index.  For FP16 the sign rides on the power-of-two scale factors.
 //
+// Out of place (the tiled passes of a long plan), every split kernel loads
+// from StockhamArgs::src at the offsets of `data` (src_dist and
+// src_dist_outer equal to dist and dist_outer, which is how those passes are
+// launched) and stores into `data`; StockhamArgs::dst is not used.
+//
 // Operand layout of v_mfma_f32_16x16x16 (A: M x K, B: K x N, D: M x N, lane l):
 //   A[i][k]: i = l % 16, k = 4 (l / 16) + r   (r = 0..3, one 4-vector per lane)
 //   B[k][n]: n = l % 16, k = 4 (l / 16) + r
@@ -307,14 +312,18 @@ __global__ void __launch_bounds__(TPT * C) c2c_split_kernel(StockhamArgs a) {
   if (COL) tile = tile_order(tile, gridDim.x, a.xcd_remap);
   const int64_t b = tile * C + c;
   const bool valid = b < a.total;
-  V2<float>* p = static_cast<V2<float>*>(a.data) + (valid ? xform_offset(b, a) : 0);
+  const int64_t off = valid ? xform_offset(b, a) : 0;
+  V2<float>* p = static_cast<V2<float>*>(a.data) + off;
+  // out of place (a long plan's tiled pass): loads from `src` at the same
+  // offsets, stores into `data`
+  const V2<float>* q = static_cast<const V2<float>*>(a.src ? a.src : a.data) + off;
   const bool sw = a.backward != 0;
   // HBM -> LDS
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int e = j + m * TPT;
     V2<float> x = mk<float>(0.f, 0.f);
-    if (valid) x = p[(int64_t)e * a.stride];
+    if (valid) x = q[(int64_t)e * a.stride];
     lds[L::at(e, c)] = sw ? swp<float>(x) : x;
   }
   __syncthreads();
