@@ -85,3 +85,42 @@ def real_supported(prec: str, n: int) -> bool:
     return (n % 2 == 0 and n // 2 in tiled_lengths()) or \
         n <= NAIVE_MAX["f64" if prec == "f64" else "f32"] or \
         (n % 2 == 0 and c2c_supported(prec, n // 2)) or (n % 2 == 1 and c2c_supported(prec, n))
+
+
+def forced_sub_plan_kernels(prec, n, batch, stride, dist, cache_path):
+    """Every kernel candidate of every tiled sub-plan of a long plan, forced
+    in turn through a tuning-cache line: yields (sub-plan key, kernel name,
+    long plan created with that line loaded).  The long plan is first
+    Measure-tuned so the cache holds one line per sub-plan geometry."""
+    import parafaft_amd as P
+    from parafaft_amd import backend as B
+    B.tune_cache_clear()
+    plan = P.Plan.c2c(prec, n, batch, stride, dist)
+    size = (batch - 1) * dist + (n - 1) * stride + 1
+    esz = np.complex128 if prec == "f64" else np.complex64
+    plan.tune(1, to_dev(np.zeros(size, esz)))
+    B.tune_cache_save(cache_path)
+    lines = [l.split("\t") for l in open(cache_path).read().splitlines()
+             if l and not l.startswith("#")]
+    assert len(lines) >= 2, lines  # one per tiled sub-plan
+    for key, val in lines:
+        # key: "p<prec> k<kind> n<length> i<inner> o<outer> s<stride> d<dist> do<dist_outer>"
+        f = {}
+        for tok in key.split():
+            name = "do" if tok.startswith("do") else tok[0]
+            f[name] = int(tok[len(name):])
+        sub = P.Plan.c2c(prec, f["n"], f["i"], f["s"], f["d"], f["o"], f["do"])
+        prefix = "|".join(val.split("|")[:4])  # precision, kind, length, layout
+        for i in range(sub.num_candidates()):
+            sub.use_candidate(i)
+            name = sub.describe()[0]
+            rad = name[name.index("_r[") + 3:name.index("]_c")]
+            per_block = int(name[name.index("]_c") + 3:].split("_")[0])
+            kid = f"{prefix}|{per_block}|{rad}|order=1"
+            with open(cache_path, "w") as fh:
+                for k2, v2 in lines:
+                    fh.write(f"{k2}\t{kid if k2 == key else v2}\n")
+            B.tune_cache_clear()
+            assert B.tune_cache_load(cache_path) == 0, (key, kid)
+            yield key, name, P.Plan.c2c(prec, n, batch, stride, dist)
+    B.tune_cache_clear()

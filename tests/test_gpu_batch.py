@@ -257,3 +257,47 @@ def test_batch_streaming_grouped(monkeypatch, prec):
         outs[mode] = to_host(d)
         _check(items, outs[mode], offs, refs, prec, cdt)
     assert np.array_equal(outs["grouped"].view(np.uint8), outs["each"].view(np.uint8))
+
+
+# radix-16 (and FP16 radix-32) transforms among the split-precision items
+SPLIT_ITEMS = [("c2c", (256, 256)), ("c2c", (1024, 64)), ("c2c", (64, 1024)),
+               ("c2c", (32, 32, 256)), ("r2c", (20, 24, 16)), ("c2c", (48, 48, 48)),
+               ("c2c", (2, 1024, 40))]
+
+
+@pytest.mark.parametrize("prec", ["f32_split_f16", "f32_split_bf16"])
+@pytest.mark.parametrize("tuned", [False, True])
+def test_batch_split_precision_vs_oracle(prec, tuned):
+    """A batch in a split precision: its matrix-core stages have no grouped
+    form and run as direct launches beside the grouped FP32 ones (lengths
+    without radix-16 passes and R2C stages use the plain kernels); grouped and
+    one launch per stage, forward and backward, every transform against the
+    oracle at the FP32 bound."""
+    from parafaft_amd.batch import Batch
+    cdt, rdt = np.complex64, np.float32
+    items = SPLIT_ITEMS
+    offs, total = _layout(items, cdt)
+    xs = _inputs(items, 70)
+    host = _pack_inputs(items, xs, offs, total, cdt, rdt)
+    refs = _refs(items, [x.astype(cdt if k == "c2c" else rdt).astype(
+        np.complex128 if k == "c2c" else np.float64) for (k, _), x in zip(items, xs)])
+    for mode in ("grouped", "each"):
+        d = to_dev(host)
+        b = Batch(prec, [(k, s, d.data_ptr() + o * 8) for (k, s), o in zip(items, offs)])
+        if tuned:
+            b.tune()
+            d.copy_(to_dev(host))
+        (b.execute if mode == "grouped" else b.execute_each)(0)
+        _check(items, to_host(d), offs, refs, "f32", cdt)
+        (b.execute if mode == "grouped" else b.execute_each)(1)
+        back = to_host(d)
+        for (kind, shape), o, x in zip(items, offs, xs):
+            N = int(np.prod(shape))
+            if kind == "c2c":
+                got = back[o:o + x.size].reshape(shape)
+            else:
+                n = shape[-1]
+                h = n // 2 + 1
+                got = back[o:o + int(np.prod(shape[:-1])) * h].view(rdt).reshape(
+                    shape[:-1] + (2 * h,))[..., :n]
+            assert rel_l2(got, N * x) <= 1e-6 * 25, (prec, mode, kind, shape)

@@ -400,3 +400,40 @@ def test_chirp_and_odd_real_batches_run_in_slices():
     assert rel_l2(to_host(d).copy().view(np.complex128), refr.view(np.complex128)) <= tol("f64", n)
     c2r.execute(1, d, scale=1.0 / n)
     assert rel_l2(to_host(d)[:, :n], real) <= 2 * tol("f64", n)
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("layout", ["row", "col"])
+@pytest.mark.parametrize("n", [8192, 1 << 20])
+def test_long_every_sub_plan_kernel(prec, layout, n, tmp_path):
+    """Every kernel candidate of both tiled sub-plans of a four-step plan
+    (8192 = 64 x 128, 2^20 = 1024 x 1024), forced in turn through a
+    tuning-cache line: the first pass runs out of place into the plan's
+    scratch, so every kernel must load from the caller's array; both
+    directions against the oracle, gaps untouched."""
+    from tests.gpu_util import forced_sub_plan_kernels
+    batch, stride, dist = _geometry(n, layout)
+    if n > 8192:
+        batch = 2
+        stride, dist = (1, n + 5) if layout == "row" else (2, 1)
+    size = (batch - 1) * dist + (n - 1) * stride + 1
+    cdt = np.complex128 if prec == "f64" else np.complex64
+    x = O.uniform_complex((size,), seed=77).astype(cdt)
+    refs = {}
+    for direction, sign in ((0, -1), (1, +1)):
+        r = x.astype(np.complex128).copy()
+        O.c2c_batched(r, n, batch, stride, dist, sign)
+        refs[direction] = r
+    touched = refs[0] != x
+    forced = 0
+    for key, name, plan in forced_sub_plan_kernels(prec, n, batch, stride, dist,
+                                                   str(tmp_path / "cache.txt")):
+        for direction in (0, 1):
+            d = to_dev(x)
+            plan.execute(direction, d)
+            got = to_host(d)
+            assert np.all(got[~touched] == x[~touched]), (key, name)
+            err = rel_l2(got, refs[direction])
+            assert err <= tol(prec, n), (prec, layout, key, name, direction, err)
+        forced += 1
+    assert forced >= 2, forced  # at least the kernel of each sub-plan

@@ -186,46 +186,18 @@ def test_split_every_kernel_in_long_plan_sub_passes(tmp_path):
     caller's array, stores into the plan's scratch).  Every split kernel of
     each sub-plan geometry, forced in turn through a tuning-cache line, must
     load from the source: the result matches the oracle."""
-    import parafaft_amd as P
-    from parafaft_amd import backend as B
+    from tests.gpu_util import forced_sub_plan_kernels
     n, batch = 1 << 20, 2
     x = O.uniform_complex((n * batch,), seed=41).astype(np.complex64)
     ref = x.astype(np.complex128).copy()
     O.c2c_batched(ref, n, batch, batch, 1, -1)
     for mode in MODES:
-        B.tune_cache_clear()
-        plan = P.Plan.c2c(mode, n, batch, batch, 1)
-        plan.tune(1, to_dev(np.zeros_like(x)))
-        path = str(tmp_path / f"cache_{mode}.txt")
-        B.tune_cache_save(path)
-        lines = [l.split("\t") for l in open(path).read().splitlines()
-                 if l and not l.startswith("#")]
-        assert len(lines) >= 2, lines  # the two tiled sub-plans
         forced = 0
-        for key, val in lines:
-            # key: "p<prec> k<kind> n<length> i<inner> o<outer> s<stride> d<dist> do<dist_outer>"
-            f = {}
-            for tok in key.split():
-                name = "do" if tok.startswith("do") else tok[0]
-                f[name] = int(tok[len(name):])
-            sub = P.Plan.c2c(mode, f["n"], f["i"], f["s"], f["d"], f["o"], f["do"])
-            prefix = "|".join(val.split("|")[:4])  # precision, kind, length, layout
-            for i in range(sub.num_candidates()):
-                sub.use_candidate(i)
-                name = sub.describe()[0]
-                rad = name[name.index("_r[") + 3:name.index("]_c")]
-                per_block = int(name[name.index("]_c") + 3:].split("_")[0])
-                kid = f"{prefix}|{per_block}|{rad}|order=1"
-                with open(path, "w") as fh:
-                    for k2, v2 in lines:
-                        fh.write(f"{k2}\t{kid if k2 == key else v2}\n")
-                B.tune_cache_clear()
-                assert B.tune_cache_load(path) == 0, (key, kid)
-                long_plan = P.Plan.c2c(mode, n, batch, batch, 1)
-                d = to_dev(x)
-                long_plan.execute(0, d)
-                err = rel_l2(to_host(d), ref)
-                assert err <= tol("f32", n), (mode, key, name, err)
-                forced += 1
+        for key, name, plan in forced_sub_plan_kernels(mode, n, batch, batch, 1,
+                                                       str(tmp_path / "cache.txt")):
+            d = to_dev(x)
+            plan.execute(0, d)
+            err = rel_l2(to_host(d), ref)
+            assert err <= tol("f32", n), (mode, key, name, err)
+            forced += 1
         assert forced >= 4, forced
-    B.tune_cache_clear()
