@@ -298,12 +298,6 @@ const char *pfx_version(void) { return "parafaft-mi355 0.1 (gfx950)"; }
 static int plan_create(pfx_plan *out, int precision, int kind, int64_t length, int64_t inner,
                        int64_t stride, int64_t dist, int64_t outer, int64_t dist_outer);
 
-// A C2C length with a tiled kernel in some layout.
-static bool has_tiled_c2c(int bp, int64_t m) {
-  return find_kernel(bp, KIND_C2C, m, false) != nullptr ||
-         find_kernel(bp, KIND_C2C, m, true) != nullptr;
-}
-
 // Releases a long plan's parts (the plan itself stays).
 static void free_long(pfx_plan p) {
   if (!p->lp) return;
@@ -327,9 +321,15 @@ static int make_long(pfx_plan p) {
   const int64_t total = p->inner * p->outer;
   int rc = PFX_SUCCESS;
   if (p->kind == PFX_C2C) {
-    // the most even split into two tiled lengths (n1 <= n2)
+    // the most even split into two tiled lengths (n1 <= n2) whose passes
+    // have kernels in the layouts they run in: the length-n2 pass (stride
+    // n1 st, dist st) is batch-contiguous when st == 1, the length-n1 pass
+    // (stride st, dist n1 st) is a row pass (some lengths have row kernels
+    // only)
     for (int64_t f = 2; f * f <= n; ++f)
-      if (n % f == 0 && has_tiled_c2c(bp, f) && has_tiled_c2c(bp, n / f)) lp->n1 = f;
+      if (n % f == 0 && find_kernel(bp, KIND_C2C, n / f, st == 1) &&
+          find_kernel(bp, KIND_C2C, f, false))
+        lp->n1 = f;
     if (lp->n1) {
       lp->mode = LONG_FOURSTEP;
       lp->n2 = n / lp->n1;

@@ -284,7 +284,12 @@ int main() {
   test_different_pointers<float>();
   test_r2c_c2r<float>();
   std::printf("lengths without a single-kernel plan\n");
-  for (int n : {100, 1000, 8191, 8192}) {
+  for (int n : {100, 1000, 8191, 32768}) {
+    test_any_length<float>(n);
+    test_any_length<double>(n);
+  }
+  std::printf("rows beyond 4096 in one kernel (FP64 16384: four-step)\n");
+  for (int n : {8192, 16384}) {
     test_any_length<float>(n);
     test_any_length<double>(n);
   }

@@ -19,11 +19,11 @@ from tests.gpu_util import long_split, rel_l2, tiled_lengths, to_dev, to_host, t
 
 pytestmark = pytest.mark.gpu
 
-CASES = [  # (prec, n, layout)
-    ("f32", 8192, "row"), ("f32", 8192, "col"), ("f64", 8192, "row"),
-    ("f32", 16384, "gen"), ("f64", 16384, "col"), ("f32", 6144, "row"),
+CASES = [  # (prec, n, layout); rows of 8192 (and FP32 16384) run one kernel
+    ("f32", 32768, "row"), ("f32", 8192, "col"), ("f64", 32768, "row"),
+    ("f32", 32768, "gen"), ("f64", 16384, "col"), ("f32", 6144, "row"),
     ("f64", 3072, "gen"), ("f32", 65536, "row"), ("f32", 12288, "col"),
-    ("f64", 10240, "row"), ("f32", 1 << 20, "row"),
+    ("f64", 10240, "row"), ("f32", 1 << 20, "row"), ("f64", 16384, "row"),
 ]
 
 
@@ -219,7 +219,8 @@ def test_odd_and_chirp_real_vs_oracle(prec, n):
 @pytest.mark.parametrize("fast", [True, False])
 def test_driver_with_a_long_axis(fast):
     """A 3 x 8192 C2C and a 2 x 16384 R2C through the one-rank drivers (the
-    backend's create_stage_plan gets the long length)."""
+    backend's create_stage_plan gets the long length: one-kernel rows, a
+    four-step batch-contiguous axis of 3 and 2)."""
     import parafaft_amd as P
     shape = [3, 8192]
     N = int(np.prod(shape))
@@ -259,8 +260,9 @@ def test_long_refusals():
     assert e.value.status == 2
     with pytest.raises(P.PfxError):
         P.Plan.r2c("f32", 2 * big, 2, 1, 2 * (big + 1))
-    n = 8192
+    n = 32768  # a four-step plan in rows (8192 and 16384 rows run one kernel)
     plan = P.Plan.c2c("f32", n, 2, 1, n)
+    assert "fourstep" in plan.describe()[0]
     d = to_dev(np.zeros(2 * n, np.complex64))
     seg = (C.c_void_p * 1)(d.data_ptr())
     st = (C.c_int * 1)(0)
@@ -407,11 +409,13 @@ def test_chirp_and_odd_real_batches_run_in_slices():
 @pytest.mark.parametrize("n", [8192, 1 << 20])
 def test_long_every_sub_plan_kernel(prec, layout, n, tmp_path):
     """Every kernel candidate of both tiled sub-plans of a four-step plan
-    (8192 = 64 x 128, 2^20 = 1024 x 1024), forced in turn through a
-    tuning-cache line: the first pass runs out of place into the plan's
-    scratch, so every kernel must load from the caller's array; both
-    directions against the oracle, gaps untouched."""
+    (8192 = 64 x 128 in columns, 32768 = 128 x 256 in rows, 2^20 = 1024 x
+    1024), forced in turn through a tuning-cache line: the first pass runs
+    out of place into the plan's scratch, so every kernel must load from the
+    caller's array; both directions against the oracle, gaps untouched."""
     from tests.gpu_util import forced_sub_plan_kernels
+    if n == 8192 and layout == "row":
+        n = 32768  # rows of 8192 run one kernel
     batch, stride, dist = _geometry(n, layout)
     if n > 8192:
         batch = 2

@@ -1,0 +1,129 @@
+"""Rows beyond 4096 points in one kernel: 8192 (FP32 and FP64) and 16384
+(FP32) in the row layout run register-resident Stockham kernels (twiddles
+from the global table, a one-component LDS exchange); batch-contiguous
+layouts of these lengths keep the four-step plan.  Every candidate kernel on
+contiguous, gapped, strided and two-level row geometries, both directions
+with a fused scale, against the oracle; elements the plan does not own stay
+untouched."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import rel_l2, to_dev, to_host, tol
+
+pytestmark = pytest.mark.gpu
+
+ROWS = [("f32", 8192), ("f64", 8192), ("f32", 16384)]
+GEOMS = {  # batch, stride, dist, outer, outer gap
+    "contiguous": (5, 1, None, 1, 0),
+    "gapped": (3, 1, 7, 2, 33),
+    "strided": (2, 3, 1, 1, 0),
+}
+
+
+def _geom(n, name):
+    batch, stride, dist, outer, gap = GEOMS[name]
+    dist = n * stride + (dist or 0) if name != "contiguous" else n
+    if name == "strided":
+        dist = 3 * (n - 1) + 1 + 5
+    span = (batch - 1) * dist + (n - 1) * stride + 1
+    outer_dist = span + gap
+    size = (outer - 1) * outer_dist + span
+    return batch, stride, dist, outer, outer_dist, size
+
+
+@pytest.mark.parametrize("prec,n", ROWS)
+@pytest.mark.parametrize("geom", list(GEOMS))
+def test_long_rows_every_candidate(prec, n, geom):
+    import parafaft_amd as P
+    batch, stride, dist, outer, outer_dist, size = _geom(n, geom)
+    cdt = np.complex128 if prec == "f64" else np.complex64
+    x = O.uniform_complex((size,), seed=n + len(geom)).astype(cdt)
+    refs = {}
+    for direction, sign in ((0, -1), (1, +1)):
+        r = x.astype(np.complex128).copy()
+        for o in range(outer):
+            v = r[o * outer_dist:].copy()
+            O.c2c_batched(v, n, batch, stride, dist, sign)
+            r[o * outer_dist:] = v
+        refs[direction] = r
+    owned = refs[0] != x
+    plan = P.Plan.c2c(prec, n, batch, stride, dist, outer, outer_dist)
+    assert plan.num_candidates() >= 1
+    names = []
+    for i in range(plan.num_candidates()):
+        plan.use_candidate(i)
+        name = plan.describe()[0]
+        names.append(name)
+        assert f"_row_n{n}_" in name, name
+        for direction, scale in ((0, 1.0), (1, 1.0 / n)):
+            d = to_dev(x)
+            plan.execute(direction, d, scale=scale)
+            got = to_host(d)
+            assert np.array_equal(got[~owned].view(np.uint8), x[~owned].view(np.uint8)), name
+            err = rel_l2(got[owned], refs[direction][owned] * scale)
+            assert err <= tol(prec, n), (name, direction, err)
+    assert len(set(names)) == len(names)
+
+
+@pytest.mark.parametrize("prec,n", ROWS)
+def test_long_rows_plan_kinds(prec, n):
+    """Rows run one tiled kernel; batch-contiguous columns of the same length
+    run the four-step plan; the tuner picks among the row kernels."""
+    import parafaft_amd as P
+    row = P.Plan.c2c(prec, n, 4, 1, n)
+    assert "fourstep" not in row.describe()[0] and f"_row_n{n}_" in row.describe()[0]
+    col = P.Plan.c2c(prec, n, 4, 4, 1)
+    assert "fourstep" in col.describe()[0], col.describe()
+    cdt = np.complex128 if prec == "f64" else np.complex64
+    row.tune(1, to_dev(np.zeros(4 * n, cdt)))
+    assert f"_row_n{n}_" in row.describe()[0]
+    x = O.uniform_complex((4 * n,), seed=3).astype(cdt)
+    ref = x.astype(np.complex128).copy()
+    O.c2c_batched(ref, n, 4, 1, n, -1)
+    d = to_dev(x)
+    row.execute(0, d)
+    assert rel_l2(to_host(d), ref) <= tol(prec, n)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 16384), ("f64", 16384), ("f32", 32768)])
+def test_long_real_rows_through_one_kernel_half(prec, n):
+    """Even real lengths whose half is a one-kernel row (8192, FP32 16384):
+    the half-length C2C runs one kernel, then the R2C post-pass; C2R back."""
+    import parafaft_amd as P
+    batch = 3
+    h = n // 2 + 1
+    pd = 2 * h
+    rdt = np.float32 if prec == "f32" else np.float64
+    real = O.uniform(batch * n, seed=n + 2).reshape(batch, n).astype(rdt)
+    pad = np.zeros((batch, pd), dtype=rdt)
+    pad[:, :n] = real
+    ref = pad.astype(np.float64).copy()
+    O.r2c_inplace(ref, n, batch, 1, pd)
+    r2c = P.Plan.r2c(prec, n, batch, 1, pd)
+    c2r = P.Plan.c2r(prec, n, batch, 1, pd)
+    d = to_dev(pad)
+    r2c.execute(0, d)
+    cdt = np.complex64 if prec == "f32" else np.complex128
+    got = to_host(d).copy().view(cdt)
+    assert rel_l2(got, ref.view(np.complex128)) <= tol(prec, n)
+    c2r.execute(1, d, scale=1.0 / n)
+    assert rel_l2(to_host(d)[:, :n], real.astype(np.float64)) <= 2 * tol(prec, n)
+
+
+@pytest.mark.parametrize("prec,n", [("f32", 4099), ("f32", 8191), ("f64", 3001)])
+def test_chirp_with_one_kernel_convolution(prec, n):
+    """Bluestein lengths whose power-of-two convolution length M is 8192 or
+    16384: the M-point transforms now run one kernel."""
+    import parafaft_amd as P
+    batch = 3
+    cdt = np.complex128 if prec == "f64" else np.complex64
+    x = O.uniform_complex((batch * n,), seed=n).astype(cdt)
+    plan = P.Plan.c2c(prec, n, batch, 1, n)
+    assert "bluestein" in plan.describe()[0]
+    for direction, sign in ((0, -1), (1, +1)):
+        ref = x.astype(np.complex128).copy()
+        O.c2c_batched(ref, n, batch, 1, n, sign)
+        d = to_dev(x)
+        plan.execute(direction, d)
+        assert rel_l2(to_host(d), ref) <= tol(prec, n)
