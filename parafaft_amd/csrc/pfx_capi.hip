@@ -321,21 +321,32 @@ static int make_long(pfx_plan p) {
   const int64_t total = p->inner * p->outer;
   int rc = PFX_SUCCESS;
   if (p->kind == PFX_C2C) {
+    // The passes' transforms: for a batch-contiguous caller (dist 1) the
+    // caller's batch is their inner index, so both passes are
+    // batch-contiguous; otherwise the inner index is j1 (length-n2 pass:
+    // stride n1 st, dist st, batch-contiguous when st == 1) or k2
+    // (length-n1 pass: stride st, dist n1 st, a row pass).
+    const bool bc = d == 1 && st > 1;
     // the most even split into two tiled lengths (n1 <= n2) whose passes
-    // have kernels in the layouts they run in: the length-n2 pass (stride
-    // n1 st, dist st) is batch-contiguous when st == 1, the length-n1 pass
-    // (stride st, dist n1 st) is a row pass (some lengths have row kernels
-    // only)
+    // have kernels in the layouts they run in (some lengths have row
+    // kernels only)
     for (int64_t f = 2; f * f <= n; ++f)
-      if (n % f == 0 && find_kernel(bp, KIND_C2C, n / f, st == 1) &&
-          find_kernel(bp, KIND_C2C, f, false))
+      if (n % f == 0 && find_kernel(bp, KIND_C2C, n / f, st == 1 || bc) &&
+          find_kernel(bp, KIND_C2C, f, bc))
         lp->n1 = f;
     if (lp->n1) {
       lp->mode = LONG_FOURSTEP;
       lp->n2 = n / lp->n1;
-      rc = plan_create(&lp->a, p->precision, PFX_C2C, lp->n2, lp->n1, lp->n1 * st, st, B, d);
-      if (rc == PFX_SUCCESS)
-        rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->n1, lp->n2, st, lp->n1 * st, B, d);
+      const int64_t n1 = lp->n1, n2 = lp->n2;
+      if (bc) {
+        rc = plan_create(&lp->a, p->precision, PFX_C2C, n2, B, n1 * st, 1, n1, st);
+        if (rc == PFX_SUCCESS)
+          rc = plan_create(&lp->c, p->precision, PFX_C2C, n1, B, st, 1, n2, n1 * st);
+      } else {
+        rc = plan_create(&lp->a, p->precision, PFX_C2C, n2, n1, n1 * st, st, B, d);
+        if (rc == PFX_SUCCESS)
+          rc = plan_create(&lp->c, p->precision, PFX_C2C, n1, n2, st, n1 * st, B, d);
+      }
       if (rc != PFX_SUCCESS) return rc;
       if (B > 0) {
         const int64_t span = (B - 1) * d + (n - 1) * st + 1;
