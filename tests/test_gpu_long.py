@@ -106,7 +106,7 @@ def test_long_c2c_measure_tuned():
     assert rel_l2(to_host(d), ref) <= tol("f32", n)
 
 
-@pytest.mark.parametrize("prec,n", [("f32", 16384), ("f64", 16384), ("f32", 12288),
+@pytest.mark.parametrize("prec,n", [("f32", 65536), ("f64", 16384), ("f32", 12288),
                                     ("f32", 1 << 18), ("f64", 8192 + 4096)])
 def test_long_r2c_c2r_vs_oracle(prec, n):
     """Even real lengths whose half has no tiled kernel: the half-length

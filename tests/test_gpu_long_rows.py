@@ -88,8 +88,9 @@ def test_long_rows_plan_kinds(prec, n):
 
 @pytest.mark.parametrize("prec,n", [("f32", 16384), ("f64", 16384), ("f32", 32768)])
 def test_long_real_rows_through_one_kernel_half(prec, n):
-    """Even real lengths whose half is a one-kernel row (8192, FP32 16384):
-    the half-length C2C runs one kernel, then the R2C post-pass; C2R back."""
+    """Even real lengths past 8192: FP32 16384 runs one R2C kernel; FP64
+    16384 and FP32 32768 run their half (8192, 16384) as a one-kernel C2C
+    row, then the R2C post-pass; C2R back."""
     import parafaft_amd as P
     batch = 3
     h = n // 2 + 1
@@ -127,3 +128,41 @@ def test_chirp_with_one_kernel_convolution(prec, n):
         d = to_dev(x)
         plan.execute(direction, d)
         assert rel_l2(to_host(d), ref) <= tol(prec, n)
+
+
+@pytest.mark.parametrize("kind", ["r2c", "c2r"])
+def test_real_16384_every_candidate(kind):
+    """Every R2C / C2R kernel of real length 16384 (FP32, half 8192 in one
+    kernel) on gapped padded rows, against the oracle; the padding and the
+    gaps stay untouched by R2C."""
+    import parafaft_amd as P
+    n, batch = 16384, 5
+    h = n // 2 + 1
+    pd = 2 * h + 6
+    real = O.uniform(batch * n, seed=17).reshape(batch, n)
+    pad = np.zeros((batch, pd))
+    pad[:, :n] = real
+    spec = pad.copy()
+    O.r2c_inplace(spec, n, batch, 1, pd)
+    if kind == "r2c":
+        src, ref = pad.astype(np.float32), spec
+        plan = P.Plan.r2c("f32", n, batch, 1, pd)
+    else:
+        back = spec.copy()
+        O.c2r_inplace(back, n, batch, 1, pd)
+        src, ref = spec.astype(np.float32), back
+        plan = P.Plan.c2r("f32", n, batch, 1, pd)
+    assert plan.num_candidates() >= 3
+    for i in range(plan.num_candidates()):
+        plan.use_candidate(i)
+        name = plan.describe()[0]
+        assert f"{kind}_row_n8192_" in name, name
+        d = to_dev(src)
+        plan.execute(0 if kind == "r2c" else 1, d)
+        got = to_host(d)
+        if kind == "r2c":
+            assert rel_l2(got[:, :2 * h].copy().view(np.complex64),
+                          ref[:, :2 * h].copy().view(np.complex128)) <= tol("f32", n), name
+            assert np.all(got[:, 2 * h:] == 0), name
+        else:
+            assert rel_l2(got[:, :n], ref[:, :n]) <= tol("f32", n), name
