@@ -170,6 +170,11 @@ struct LongPlan {
   int64_t rows = 0;         // LONG_CHIRP / LONG_REAL_FULL: scratch rows (a slice of the batch)
   pfx_plan_t *a = nullptr;
   pfx_plan_t *c = nullptr;
+  // four-step, batch-contiguous caller: the fused first pass (its per-pass
+  // table `ptw1`) and the second pass `c2`, instead of a, twiddle, c and
+  // the transpose
+  pfx_plan_t *c2 = nullptr;
+  void *ptw1 = nullptr;
   void *scratch = nullptr;  // the mode's work rows (four-step: one outer
                             // slice's footprint of the caller's layout)
   void *tw = nullptr;       // four-step W_n^i, i < n; real halves W_n^k; chirp w
@@ -303,6 +308,8 @@ static void free_long(pfx_plan p) {
   if (!p->lp) return;
   pfx_plan_destroy(p->lp->a);
   pfx_plan_destroy(p->lp->c);
+  pfx_plan_destroy(p->lp->c2);
+  if (p->lp->ptw1) hipFree(p->lp->ptw1);
   if (p->lp->scratch) hipFree(p->lp->scratch);
   if (p->lp->tw) hipFree(p->lp->tw);
   if (p->lp->filt) hipFree(p->lp->filt);
@@ -338,7 +345,16 @@ static int make_long(pfx_plan p) {
       lp->mode = LONG_FOURSTEP;
       lp->n2 = n / lp->n1;
       const int64_t n1 = lp->n1, n2 = lp->n2;
-      if (bc) {
+      Radices rad1;
+      if (bc && (p->precision == PFX_F32 || p->precision == PFX_F64) &&
+          fourstep_cols_pass1_radices(n2, &rad1)) {
+        // two passes: the fused first pass, then the length-n1 DFTs from
+        // the scratch ((n2 j1 + k2) st + b) into the caller's array
+        rc = plan_create(&lp->c2, p->precision, PFX_C2C, n1, B, n2 * st, 1, n2, st);
+        if (rc != PFX_SUCCESS) return rc;
+        PFX_HIP(hipMalloc(&lp->ptw1, (size_t)n2 * esz), "hipMalloc four-step pass table");
+        PFX_HIP(make_pass_twiddles(bp, rad1, lp->ptw1, nullptr), "four-step pass table");
+      } else if (bc) {
         rc = plan_create(&lp->a, p->precision, PFX_C2C, n2, B, n1 * st, 1, n1, st);
         if (rc == PFX_SUCCESS)
           rc = plan_create(&lp->c, p->precision, PFX_C2C, n1, B, st, 1, n2, n1 * st);
@@ -567,8 +583,9 @@ static int execute_impl(pfx_plan p, int direction, void *data, double scale, pfx
                         void *real_oop, int64_t real_dist);
 
 // A C2C tiled plan from `src` into `dst` at the same offsets (src null: in
-// place on dst), scale 1.
-static int launch_tiled(pfx_plan p, int direction, void *dst, const void *src, hipStream_t s) {
+// place on dst), outputs times `scale`.
+static int launch_tiled(pfx_plan p, int direction, void *dst, const void *src, hipStream_t s,
+                        double scale = 1.0) {
   if (!p->kernel) return fail(PFX_ERR_INTERNAL, "sub-plan without a tiled kernel");
   StockhamArgs a;
   a.data = dst;
@@ -582,8 +599,8 @@ static int launch_tiled(pfx_plan p, int direction, void *dst, const void *src, h
   a.stride = p->stride;
   a.dist = p->dist;
   a.dist_outer = p->dist_outer;
-  a.scale = 1.0;
-  a.use_scale = 0;
+  a.scale = scale;
+  a.use_scale = scale != 1.0;
   a.backward = direction == PFX_BACKWARD;
   a.xcd_remap = p->xcd_remap;
   a.in_seg = a.out_seg = 0;
@@ -654,6 +671,23 @@ static int run_long(pfx_plan p, int direction, void *data, double scale, hipStre
     fa.backward = direction == PFX_BACKWARD;
     for (int64_t o = 0; o < p->outer; ++o) {
       char *base = static_cast<char *>(data) + (size_t)(o * p->dist_outer) * esz;
+      if (lp->c2) {
+        FsColArgs ca;
+        ca.src = base;
+        ca.dst = lp->scratch;
+        ca.ptw = lp->ptw1;
+        ca.tw = lp->tw;
+        ca.n1 = lp->n1;
+        ca.n2 = lp->n2;
+        ca.st = p->stride;
+        ca.batch = p->inner;
+        ca.tiles_per_j1 = 0;
+        ca.backward = direction == PFX_BACKWARD;
+        PFX_HIP(launch_fourstep_cols_pass1(bp, ca, s), "four-step first pass");
+        const int rc = launch_tiled(lp->c2, direction, base, lp->scratch, s, scale);
+        if (rc != PFX_SUCCESS) return rc;
+        continue;
+      }
       int rc = launch_tiled(lp->a, direction, lp->scratch, base, s);
       if (rc != PFX_SUCCESS) return rc;
       fa.use_scale = 0;
@@ -875,6 +909,8 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
       int rc = pfx_plan_tune(p->lp->a, flag, scratch, stream);
       if (rc != PFX_SUCCESS) return rc;
     }
+    if (p->lp->c2)  // in place on the caller's scratch: its footprint is the plan's
+      return pfx_plan_tune(p->lp->c2, flag, scratch, stream);
     // `c` works on the plan's own rows except for the real halves (the
     // caller's rows read as complex pairs)
     if (p->lp->mode != LONG_REAL_HALF && !p->lp->scratch) return PFX_SUCCESS;  // empty batch

@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "pfx_internal.hpp"
+#include "pfx_tiles.hpp"
 
 namespace pfx {
 
@@ -305,6 +306,89 @@ __global__ void __launch_bounds__(256) herm_full_store(const V2<T>* scr, T* rows
   }
 }
 
+// Fused first pass of the four-step plan for batch-contiguous callers
+// (element k of transform b at k st + b): the length-n2 DFTs along j2 of
+// x[j1 + n1 j2], times the twiddle W_n^(+-j1 k2), stored at (n2 j1 + k2) st
+// + b of the scratch.  The second pass (the length-n1 DFTs along j1, stride
+// n2 st: an ordinary out-of-place batch-contiguous plan) reads them there
+// and writes X[n2 k1 + k2] into the caller's array at the same offsets, so
+// the plan makes two passes instead of four.  A tile is C consecutive
+// transforms b of one j1: its loads and stores run along the contiguous
+// batch axis.  The passes run in LDS (Stockham, natural order in and out).
+template <typename T, int N, int TPT, int C, int Ns, int... Rs> struct FsPasses;
+template <typename T, int N, int TPT, int C, int Ns> struct FsPasses<T, N, TPT, C, Ns> {
+  __device__ static __forceinline__ void run(int, int, V2<T>*, const V2<T>*) {}
+};
+template <typename T, int N, int TPT, int C, int Ns, int R, int... Rest>
+struct FsPasses<T, N, TPT, C, Ns, R, Rest...> {
+  using L = LdsLayout<T, N, C, true>;
+  struct StoreLds {
+    V2<T>* lds;
+    int c;
+    __device__ void before() const { __syncthreads(); }
+    __device__ void after() const { __syncthreads(); }
+    __device__ void operator()(int e, V2<T> v) const { lds[L::at(e, c)] = v; }
+  };
+  __device__ static __forceinline__ void run(int j, int c, V2<T>* lds, const V2<T>* stw) {
+    auto load = [&](int e) -> V2<T> { return lds[L::at(e, c)]; };
+    stockham_pass<T, N, R, Ns, TPT>(j, stw, load, StoreLds{lds, c});
+    FsPasses<T, N, TPT, C, Ns * R, Rest...>::run(j, c, lds, stw);
+  }
+};
+
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) fourstep_cols_pass1(FsColArgs a) {
+  constexpr int N = (Rs * ... * 1);  // n2
+  constexpr int E = N / TPT;
+  static_assert(N % TPT == 0, "whole elements per thread");
+  using L = LdsLayout<T, N, C, true>;
+  using B = LdsBlock<T, V2<T>, N, L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<T>* lds = B::data(smem);
+  const V2<T>* stw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(B::tw(smem), a.ptw);
+  int j, c;
+  thread_coords<TPT, C, true>(j, c);
+  const int64_t j1 = blockIdx.x / a.tiles_per_j1;
+  const int64_t b = (blockIdx.x - j1 * a.tiles_per_j1) * C + c;
+  const bool valid = b < a.batch;
+  const bool bw = a.backward != 0;
+  const V2<T>* src = static_cast<const V2<T>*>(a.src) + (j1 * a.st + (valid ? b : 0));
+  const int64_t ls = a.n1 * a.st;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int e = j + m * TPT;
+    V2<T> x = mk<T>(T(0), T(0));
+    if (valid) x = src[e * ls];
+    lds[L::at(e, c)] = bw ? swp<T>(x) : x;
+  }
+  __syncthreads();
+  FsPasses<T, N, TPT, C, 1, Rs...>::run(j, c, lds, stw);
+  V2<T>* dst = static_cast<V2<T>*>(a.dst) + ((int64_t)N * j1 * a.st + (valid ? b : 0));
+  const V2<T>* w = static_cast<const V2<T>*>(a.tw);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k2 = j + m * TPT;
+    V2<T> y = lds[L::at(k2, c)];
+    if (bw) y = swp<T>(y);
+    const V2<T> t = w[j1 * k2];  // j1 k2 <= (n1 - 1)(n2 - 1) < n
+    y = bw ? cmulc<T>(y, t) : cmul<T>(y, t);
+    if (valid) dst[k2 * a.st] = y;
+  }
+}
+
+template <typename T, int TPT, int... Rs>
+void launch_cols_pass1(const FsColArgs& a, hipStream_t s) {
+  constexpr int N = (Rs * ... * 1);
+  // two workgroups per CU up to 512 points (16 columns, 128-byte runs);
+  // beyond, one with the CU's whole LDS keeps 8-16 columns (32-column tiles
+  // of the short lengths measured no faster)
+  constexpr int C = col_tile<N, TPT, (int)sizeof(V2<T>), (N >= 1024 ? kLdsBig : kLdsBudget)>();
+  FsColArgs b = a;
+  b.tiles_per_j1 = (a.batch + C - 1) / C;
+  const int64_t blocks = a.n1 * b.tiles_per_j1;
+  fourstep_cols_pass1<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(b);
+}
+
 int grid_for(int64_t work) {
   const int64_t blocks = (work + 255) / 256;
   return (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
@@ -450,6 +534,43 @@ hipError_t launch_real_full(int prec, int which, void* rows, void* scr, const Re
       default: herm_full_store<float><<<g, 256, 0, s>>>(c, r, a); break;
     }
   }
+  return hipGetLastError();
+}
+
+// The fused first pass for n2 (radices of the length's default plan), both
+// precisions; false when n2 has none (the plan then makes four passes).
+namespace {
+struct ColsPass1 {
+  int n2;
+  Radices rad;
+  void (*f32)(const FsColArgs&, hipStream_t);
+  void (*f64)(const FsColArgs&, hipStream_t);
+};
+#define PFX_FS_COLS(TPT, ...)                                                               \
+  {prod<__VA_ARGS__>(), radices<__VA_ARGS__>(), &launch_cols_pass1<float, TPT, __VA_ARGS__>, \
+   &launch_cols_pass1<double, TPT, __VA_ARGS__>},
+const ColsPass1 kColsPass1[] = {
+    PFX_FS_COLS(8, 8, 8) PFX_FS_COLS(8, 16, 8) PFX_FS_COLS(16, 16, 16)
+    PFX_FS_COLS(64, 8, 8, 8) PFX_FS_COLS(64, 16, 16, 4) PFX_FS_COLS(128, 16, 16, 8)};
+#undef PFX_FS_COLS
+const ColsPass1* cols_pass1(int64_t n2) {
+  for (const ColsPass1& e : kColsPass1)
+    if (e.n2 == n2) return &e;
+  return nullptr;
+}
+}  // namespace
+
+bool fourstep_cols_pass1_radices(int64_t n2, Radices* rad) {
+  const ColsPass1* e = cols_pass1(n2);
+  if (e && rad) *rad = e->rad;
+  return e != nullptr;
+}
+
+hipError_t launch_fourstep_cols_pass1(int prec, const FsColArgs& a, hipStream_t s) {
+  const ColsPass1* e = cols_pass1(a.n2);
+  if (!e) return hipErrorInvalidValue;
+  if (a.n1 * a.batch == 0) return hipSuccess;
+  (prec ? e->f64 : e->f32)(a, s);
   return hipGetLastError();
 }
 

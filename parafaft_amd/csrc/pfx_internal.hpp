@@ -146,6 +146,22 @@ hipError_t launch_fourstep_twiddle(int prec, void *buf, const void *tw, const Fo
 // dst[n2 k1 + k2] = scale * src[k1 + n1 k2]
 hipError_t launch_fourstep_transpose(int prec, const void *src, void *dst,
                                      const FourStepArgs &a, hipStream_t s);
+// Fused first pass of the four-step plan for batch-contiguous callers
+// (element k of transform b at k st + b): dst[(n2 j1 + k2) st + b] =
+// W_n^(+-j1 k2) * DFT_n2 over j2 of src[(j1 + n1 j2) st + b]; tw: W_n^i,
+// i < n; ptw: the per-pass table of the n2-point radix plan.
+struct FsColArgs {
+  const void *src;
+  void *dst;
+  const void *ptw;
+  const void *tw;
+  int64_t n1, n2, st, batch;
+  int64_t tiles_per_j1;  // set by the launcher
+  int backward;
+};
+// The radices of the fused first pass for n2; false: none for this n2.
+bool fourstep_cols_pass1_radices(int64_t n2, Radices *rad);
+hipError_t launch_fourstep_cols_pass1(int prec, const FsColArgs &a, hipStream_t s);
 // Long R2C post-pass / C2R pre-pass on padded rows (row b at b * dist complex
 // slots, h + 1 slots each); tw: W_2h^k, k <= h.
 struct RealPassArgs {

@@ -440,4 +440,4 @@ def test_long_every_sub_plan_kernel(prec, layout, n, tmp_path):
             err = rel_l2(got, refs[direction])
             assert err <= tol(prec, n), (prec, layout, key, name, direction, err)
         forced += 1
-    assert forced >= 2, forced  # at least the kernel of each sub-plan
+    assert forced >= 1, forced  # at least the kernel of each tiled sub-plan
