@@ -166,3 +166,47 @@ def test_real_16384_every_candidate(kind):
             assert np.all(got[:, 2 * h:] == 0), name
         else:
             assert rel_l2(got[:, :n], ref[:, :n]) <= tol("f32", n), name
+
+
+@pytest.mark.parametrize("case", [
+    ("c2c", "f32", 32768, 2, 1, 32768),   # four-step rows (four passes)
+    ("c2c", "f32", 8192, 6, 6, 1),        # four-step columns (fused, two passes)
+    ("c2c", "f64", 8192, 3, 1, 8192),     # one-kernel rows
+    ("c2c", "f32", 8191, 3, 1, 8191),     # chirp convolution
+    ("r2c", "f32", 32768, 2, 1, 32770),   # real: half C2C + post-pass
+])
+def test_long_plans_under_graph_capture(case):
+    """A long plan's execution captured into a HIP graph and replayed several
+    times, with eager work and synchronisations between the replays: every
+    replay equals the eager result bit for bit (the plan's scratch and tables
+    are allocated at creation; no allocation or memset is captured)."""
+    import parafaft_amd as P
+    from parafaft_amd.batch import Graph
+    from tests.gpu_util import torch
+    t = torch()
+    kind, prec, n, batch, stride, dist = case
+    stream = t.cuda.Stream()
+    if kind == "c2c":
+        cdt = np.complex128 if prec == "f64" else np.complex64
+        size = (batch - 1) * dist + (n - 1) * stride + 1
+        host = O.uniform_complex((size,), seed=n + batch).astype(cdt)
+        plan = P.Plan.c2c(prec, n, batch, stride, dist)
+        direction = 0
+    else:
+        host = np.zeros((batch, dist), np.float32)
+        host[:, :n] = O.uniform(batch * n, seed=n).reshape(batch, n)
+        host = host.reshape(-1)
+        plan = P.Plan.r2c(prec, n, batch, 1, dist)
+        direction = 0
+    d = to_dev(host)
+    plan.execute(direction, d, stream=stream.cuda_stream)
+    stream.synchronize()
+    eager = to_host(d)
+    g = Graph(stream.cuda_stream, lambda: plan.execute(direction, d, stream=stream.cuda_stream))
+    for rep in range(4):
+        d.copy_(t.from_numpy(host).to("cuda"))
+        t.cuda.synchronize()
+        _ = (t.ones(1000, device="cuda") * 2).sum().item()  # eager work between replays
+        g.launch(stream.cuda_stream)
+        stream.synchronize()
+        assert np.array_equal(to_host(d).view(np.uint8), eager.view(np.uint8)), (case, rep)
