@@ -210,3 +210,49 @@ def test_long_plans_under_graph_capture(case):
         g.launch(stream.cuda_stream)
         stream.synchronize()
         assert np.array_equal(to_host(d).view(np.uint8), eager.view(np.uint8)), (case, rep)
+
+
+@pytest.mark.parametrize("kind,prec,shape", [
+    ("c2c", "f32", (3, 8192, 5)),     # long middle axis: two-level batch-contiguous four-step
+    ("c2c", "f64", (8192, 6)),        # long outer axis, batch-contiguous
+    ("c2c", "f32", (2, 5, 16384)),    # long rows in one kernel
+    ("c2c", "f32", (32768, 3)),       # long outer axis beyond one kernel
+    ("r2c", "f32", (4, 8192, 10)),    # R2C with a long middle C2C axis
+    ("r2c", "f64", (3, 16384)),       # long real rows (FP64: half 8192 as one C2C kernel)
+])
+@pytest.mark.parametrize("fast", [True, False])
+def test_long_axes_in_nd_arrays(kind, prec, shape, fast):
+    """Long axes anywhere in an N-D array through both one-rank driver modes:
+    forward against the oracle, backward with the fused 1/N scale back to the
+    input."""
+    import parafaft_amd as P
+    shape = list(shape)
+    N = int(np.prod(shape))
+    t = tol(prec, N)
+    if kind == "c2c":
+        cdt = np.complex64 if prec == "f32" else np.complex128
+        x = O.uniform_complex(tuple(shape), seed=N % 1000).astype(cdt)
+        ref = O.fftn(x.astype(np.complex128))
+        fft = P.ParaFaFT_C2C(shape, precision=prec, fast=fast)
+        d = to_dev(x)
+        fft.forward(d)
+        assert rel_l2(to_host(d), ref) <= t
+        fft.backward(d, scale=1.0 / N)
+        assert rel_l2(to_host(d), x) <= 2 * t
+    else:
+        rdt = np.float32 if prec == "f32" else np.float64
+        cdt = np.complex64 if prec == "f32" else np.complex128
+        x = O.uniform(N, seed=N % 997).reshape(shape).astype(rdt)
+        ref = O.rfftn_padded(x.astype(np.float64))
+        fft = P.ParaFaFT_R2C(shape, precision=prec, fast=fast)
+        h = shape[-1] // 2 + 1
+        rows = int(np.prod(shape[:-1]))
+        pad = np.zeros(fft.get_required_output_size(), rdt)
+        pad[:rows * 2 * h].reshape(rows, 2 * h)[:, :shape[-1]] = x.reshape(rows, -1)
+        d = to_dev(pad)
+        fft.forward_in_place(d)
+        got = to_host(d)[:rows * 2 * h].copy().view(cdt).reshape(shape[:-1] + [h])
+        assert rel_l2(got, ref) <= t
+        fft.backward_in_place(d, scale=1.0 / N)
+        back = to_host(d)[:rows * 2 * h].reshape(rows, 2 * h)[:, :shape[-1]]
+        assert rel_l2(back.reshape(shape), x) <= 2 * t
