@@ -59,11 +59,27 @@ __device__ __forceinline__ float bf16_to_f(short h) {
   return __uint_as_float(((uint32_t)(uint16_t)h) << 16);
 }
 
+// max(v of lane l, v of lane l ^ 32) in every lane, by v_permlane32_swap
+// (lanes 32-63 of one copy trade places with lanes 0-31 of the other, so the
+// two results hold the pair in some order): a cross-lane move in the VALU,
+// no LDS round trip.
+__device__ __forceinline__ float max_lane_xor32(float v) {
+  const auto r =
+      __builtin_amdgcn_permlane32_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+// the same for lane l ^ 16 (v_permlane16_swap: odd 16-lane rows of one copy
+// trade places with even rows of the other)
+__device__ __forceinline__ float max_lane_xor16(float v) {
+  const auto r =
+      __builtin_amdgcn_permlane16_swap(__float_as_uint(v), __float_as_uint(v), false, false);
+  return fmaxf(__uint_as_float(r[0]), __uint_as_float(r[1]));
+}
+
 // Largest value over the 16 inputs of the butterfly in lane column
 // u = lane % 16 (lanes u, u + 16, u + 32, u + 48 hold them).
 __device__ __forceinline__ float butterfly_max(float v) {
-  v = fmaxf(v, __shfl_xor(v, 16, 64));
-  return fmaxf(v, __shfl_xor(v, 32, 64));
+  return max_lane_xor32(max_lane_xor16(v));
 }
 
 // cos(2 pi t / 16), t = 0..15, by selects (no indexed table, which would

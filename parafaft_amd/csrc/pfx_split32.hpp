@@ -138,6 +138,8 @@ template <> struct MfmaDft32<SPLIT_F16> {
     float amax = 0.f;
 #pragma unroll
     for (int r = 0; r < 16; ++r) amax = fmaxf(amax, fmaxf(fabsf(x[r].x), fabsf(x[r].y)));
+    // (ds_bpermute: v_permlane32_swap needs a second register here, where
+    // the kernel sits at its 256-VGPR limit, and spilled one)
     amax = fmaxf(amax, __shfl_xor(amax, 32, 64));
     // amax < 2^ex; the scaled maximum stays below 2^15 (radix-16 form)
     const int ex = (int)(__float_as_uint(amax) >> 23) - 126;
