@@ -58,11 +58,11 @@ const char *pfx_version(void);
  * share one plan.
  * Lengths: one tiled kernel where the length has a radix plan (up to 4096);
  * else a four-step plan when the length splits into two lengths with radix
- * plans (n = n1 * n2, e.g. 100 = 10 * 10, 8192 = 64 * 128, up to 4096^2):
+ * plans (n = n1 * n2, e.g. 144 = 12 * 12, 8192 = 64 * 128, up to 4096^2):
  * two tiled passes, a twiddle pass and a transpose through a scratch buffer
  * of the plan's footprint; any other length up to 2^23 through Bluestein's
  * chirp convolution (zero-padded rows of M >= 2n - 1, two M-point
- * transforms).  These plans own their scratch, so executions of one such
+ * transforms; M the shortest 2^k, 3 2^k, 5 2^k, ... with a one-kernel plan).  These plans own their scratch, so executions of one such
  * plan must not overlap.  Longer lengths that do not split:
  * PFX_ERR_UNSUPPORTED.  (The direct O(n^2) DFT, up to 4096 complex<float> /
  * 2048 complex<double>, remains for pfx_execute_c2c_noalloc.)

@@ -318,6 +318,34 @@ static void free_long(pfx_plan p) {
 }
 
 // Sub-plans, scratch and twiddles of a long plan (see LongPlan).
+// Convolution length of a chirp plan over n points: the shortest M >= 2n - 1
+// of the forms b 2^k (b = 1, 3, 5, 15, 25, 125) that runs as one row
+// kernel; else the shortest such M that splits into two tiled lengths (the
+// length-M/f pass of contiguous rows runs batch-contiguous, the length-f
+// pass in rows).  A one-kernel M at most twice the shortest costs less than
+// any four-step M.  2^k always qualifies up to 4096^2.
+static int64_t chirp_length(int bp, int64_t n) {
+  static const int64_t kBase[] = {1, 3, 5, 15, 25, 125};
+  const int64_t need = 2 * n - 1;
+  int64_t one = 0, split = 0;
+  for (const int64_t b : kBase) {
+    int64_t m = b;
+    while (m < need) m <<= 1;
+    if (find_kernel(bp, KIND_C2C, m, false)) {
+      if (!one || m < one) one = m;
+      continue;
+    }
+    if (split && m >= split) continue;
+    for (int64_t f = 2; f * f <= m; ++f)
+      if (m % f == 0 && find_kernel(bp, KIND_C2C, m / f, true) &&
+          find_kernel(bp, KIND_C2C, f, false)) {
+        split = m;
+        break;
+      }
+  }
+  return one ? one : split;
+}
+
 static int make_long(pfx_plan p) {
   const int bp = base_prec(p->precision);
   const size_t esz = bp ? 16 : 8;
@@ -376,8 +404,7 @@ static int make_long(pfx_plan p) {
                                          "chirp-convolution limit of %lld",
                     (long long)kChirpMaxLength);
       lp->mode = LONG_CHIRP;
-      lp->M = 1;
-      while (lp->M < 2 * n - 1) lp->M <<= 1;
+      lp->M = chirp_length(bp, n);
       lp->rows = std::max<int64_t>(1, std::min<int64_t>(total, kLongRowsBytes / (lp->M * (int64_t)esz)));
       rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->M, lp->rows, 1, lp->M, 1, 0);
       if (rc != PFX_SUCCESS) return rc;

@@ -73,6 +73,39 @@ def long_split(n: int):
     return best
 
 
+# Row lengths beyond the tiled plans that still run one kernel.
+LONG_ROW_ONE_KERNEL = {"f32": {8192, 16384}, "f64": {8192}}
+
+
+def _split_ok(m: int) -> bool:
+    t = tiled_lengths()
+    f = 2
+    while f * f <= m:
+        if m % f == 0 and f in t and m // f in t:
+            return True
+        f += 1
+    return False
+
+
+def chirp_length(prec: str, n: int) -> int:
+    """Convolution length of a chirp plan: the shortest M >= 2n - 1 of the
+    forms b 2^k (b = 1, 3, 5, 15, 25, 125) with a one-kernel row plan, else
+    the shortest such M that splits into two tiled lengths."""
+    base = "f64" if prec == "f64" else "f32"
+    one = tiled_lengths() | LONG_ROW_ONE_KERNEL[base]
+    need = 2 * n - 1
+    cands = []
+    for b in (1, 3, 5, 15, 25, 125):
+        m = b
+        while m < need:
+            m *= 2
+        cands.append(m)
+    ones = [m for m in cands if m in one]
+    if ones:
+        return min(ones)
+    return min(m for m in cands if _split_ok(m))
+
+
 CHIRP_MAX = 1 << 23   # Bluestein: the M >= 2n - 1 convolution must split (M <= 4096^2)
 
 

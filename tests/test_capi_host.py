@@ -146,6 +146,23 @@ def test_long_split_rule():
     assert real_supported("f64", 4097) and not real_supported("f32", 2 * ((1 << 23) + 1))
 
 
+def test_chirp_length_rule():
+    """The chirp plans' convolution length the tests expect (the library's
+    rule in chirp_length): one-kernel lengths of the forms b 2^k first,
+    then the shortest that splits."""
+    import sys
+    sys.path.insert(0, ROOT)
+    from tests.gpu_util import chirp_length
+    assert chirp_length("f32", 1100) == 2560      # 5 * 2^9 instead of 4096
+    assert chirp_length("f32", 1001) == 2048
+    assert chirp_length("f32", 700) == 1536
+    assert chirp_length("f32", 4099) == 16384     # one kernel beats a four-step 12288
+    assert chirp_length("f64", 4099) == 10240     # no one-kernel FP64 16384: shortest split
+    assert chirp_length("f32", 10007) == 20480
+    assert chirp_length("f64", 7000) == 15360
+    assert chirp_length("f32", (1 << 23)) == 1 << 24
+
+
 def test_tune_cache_load_counts_dropped_lines(tmp_path):
     """Malformed keys and kernels this build lacks are dropped and counted
     (host-only: no plan is created)."""

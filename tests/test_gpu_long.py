@@ -15,14 +15,15 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from tests.gpu_util import long_split, rel_l2, tiled_lengths, to_dev, to_host, tol
+from tests.gpu_util import (chirp_length, long_split, rel_l2, tiled_lengths, to_dev, to_host,
+                            tol)
 
 pytestmark = pytest.mark.gpu
 
 CASES = [  # (prec, n, layout); rows of 8192 (and FP32 16384) run one kernel
     ("f32", 32768, "row"), ("f32", 8192, "col"), ("f64", 32768, "row"),
     ("f32", 32768, "gen"), ("f64", 16384, "col"), ("f32", 6144, "row"),
-    ("f64", 3072, "gen"), ("f32", 65536, "row"), ("f32", 12288, "col"),
+    ("f64", 5120, "gen"), ("f32", 65536, "row"), ("f32", 12288, "col"),
     ("f64", 10240, "row"), ("f32", 1 << 20, "row"), ("f64", 16384, "row"),
 ]
 
@@ -145,7 +146,7 @@ def test_long_r2c_c2r_vs_oracle(prec, n):
 
 CHIRP_CASES = [  # lengths that split into no tiled pair: Bluestein
     ("f32", 8191, "row"), ("f32", 4099, "col"), ("f64", 2053, "gen"),
-    ("f32", 10007, "gen"), ("f64", 5000, "row"), ("f32", 5000, "col"),
+    ("f32", 10007, "gen"), ("f64", 7000, "row"), ("f32", 7000, "col"),
 ]
 
 
@@ -166,9 +167,7 @@ def test_chirp_c2c_vs_oracle(case):
     idx = (np.arange(batch)[:, None] * dist + np.arange(n)[None, :] * stride).ravel()
     owned[idx] = True
     plan = P.Plan.c2c(prec, n, batch, stride, dist)
-    M = 1
-    while M < 2 * n - 1:
-        M *= 2
+    M = chirp_length(prec, n)
     assert plan.describe()[0].endswith(f"bluestein_n{n}_m{M}"), plan.describe()
     for direction, sign, scale in ((0, -1, 1.0), (1, +1, 1.0 / n)):
         ref = x.astype(np.complex128).copy()
@@ -283,7 +282,7 @@ def test_long_refusals():
         b.execute_persistent(0)
 
 
-@pytest.mark.parametrize("prec,n", [("f32", 7), ("f32", 100), ("f32", 4093), ("f64", 2047),
+@pytest.mark.parametrize("prec,n", [("f32", 7), ("f32", 144), ("f32", 4093), ("f64", 2047),
                                     ("f64", 99)])
 def test_direct_dft_kept_under_the_knob(prec, n, monkeypatch):
     """PFX_LONG_MIN (a measurement knob) keeps the direct O(n^2) DFT below
@@ -322,7 +321,7 @@ def test_direct_dft_kept_under_the_knob(prec, n, monkeypatch):
     assert rel_l2(to_host(d)[:, :n], real.astype(np.float64)) <= 2 * tol(prec, n)
 
 
-@pytest.mark.parametrize("prec,n", [("f32", 7), ("f32", 100), ("f64", 99), ("f32", 1000)])
+@pytest.mark.parametrize("prec,n", [("f32", 7), ("f32", 144), ("f64", 99), ("f32", 1001)])
 def test_short_lengths_without_a_kernel_run_long_plans(prec, n):
     """By default a length with no tiled kernel runs a long plan (four-step
     when it splits into two tiled lengths, else the chirp convolution), not
@@ -373,11 +372,11 @@ def test_random_lengths_vs_oracle(case):
 
 def test_chirp_and_odd_real_batches_run_in_slices():
     """Batches whose chirp (odd-real) scratch rows exceed 256 MB run in
-    slices of rows through one bounded scratch: 20000 transforms of 1000
+    slices of rows through one bounded scratch: 20000 transforms of 1001
     points in columns (M = 2048: two slices, the second short) and 20000
     odd real rows of 1001 (FP64), against the oracle."""
     import parafaft_amd as P
-    n, batch = 1000, 20000
+    n, batch = 1001, 20000
     x = O.uniform_complex((n * batch,), seed=31).astype(np.complex64)
     plan = P.Plan.c2c("f32", n, batch, batch, 1)
     assert "bluestein" in plan.describe()[0]

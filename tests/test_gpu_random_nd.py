@@ -13,7 +13,7 @@ from tests.gpu_util import rel_l2, to_dev, to_host, tol
 pytestmark = pytest.mark.gpu
 
 _EXTENTS = [2, 3, 4, 5, 6, 7, 8, 10, 12, 15, 16, 18, 20, 24, 30, 32, 36, 40, 48, 60, 64, 80,
-            96, 100, 120, 128, 160, 192, 256]
+            96, 100, 120, 128, 160, 192, 256, 384, 640, 1000]
 
 
 def _cases(seed, count):

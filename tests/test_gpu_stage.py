@@ -14,9 +14,10 @@ from tests.gpu_util import real_supported, rel_l2, to_dev, to_host, tol
 
 pytestmark = pytest.mark.gpu
 
-TILED = [2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 32, 40, 48, 60, 64, 80, 96, 120, 128, 160,
-         192, 240, 256, 320, 512, 1024, 2048, 4096]
-FALLBACK = [1, 7, 11, 100, 97, 1000]
+TILED = [2, 3, 4, 5, 6, 8, 10, 12, 15, 16, 20, 24, 32, 40, 48, 60, 64, 80, 96, 100, 120, 128,
+         160, 192, 240, 256, 320, 384, 512, 640, 768, 1000, 1024, 1280, 1536, 2048, 2560, 3072,
+         4096]
+FALLBACK = [1, 7, 11, 144, 97, 1001]
 PRECS = ["f32", "f64"]
 
 
@@ -59,7 +60,8 @@ def test_contiguous_rows(prec, n):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-@pytest.mark.parametrize("n", [2, 3, 8, 16, 48, 64, 80, 120, 256, 1024, 2048, 4096, 7, 100])
+@pytest.mark.parametrize("n", [2, 3, 8, 16, 48, 64, 80, 120, 256, 1024, 2048, 4096, 7, 100, 144,
+                               384, 640, 1000, 1536, 2560, 3072])
 @pytest.mark.parametrize("batch", [1, 5, 16, 67])
 def test_batch_contiguous_columns(prec, n, batch):
     """Axis-0 layout: stride = batch, dist = 1 (parafaft_c2c.hpp:264-272)."""
@@ -124,7 +126,7 @@ def test_same_plan_other_pointer(prec):
 
 @pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("n", [4, 6, 8, 10, 16, 30, 32, 64, 80, 96, 120, 160, 256, 1024, 2048,
-                               4096, 8192, 2, 3, 5, 7, 15, 99, 100])
+                               4096, 8192, 2, 3, 5, 7, 15, 99, 100, 200, 768, 2000, 3072, 6144])
 def test_r2c_c2r_inplace(prec, n):
     """In-place padded R2C / C2R (fft_backend_fftw.hpp:439-549)."""
     import parafaft_amd as P

@@ -51,7 +51,7 @@ def test_noalloc_direct_dft_for_lengths_without_a_kernel():
     path takes a long plan, which needs scratch): against the oracle."""
     from parafaft_amd import _capi as A
     L = A.load()
-    for prec, n, batch in (("f32", 100, 3), ("f64", 7, 2)):
+    for prec, n, batch in (("f32", 144, 3), ("f64", 7, 2)):
         cdt = np.complex64 if prec == "f32" else np.complex128
         x = O.uniform_complex((batch * n,), seed=n).astype(cdt)
         d = to_dev(x)
