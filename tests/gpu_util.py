@@ -74,7 +74,7 @@ def long_split(n: int):
 
 
 # Row lengths beyond the tiled plans that still run one kernel.
-LONG_ROW_ONE_KERNEL = {"f32": {8192, 16384}, "f64": {8192}}
+LONG_ROW_ONE_KERNEL = {"f32": {8192, 16384}, "f64": {8192, 16384}}
 
 
 def _split_ok(m: int) -> bool:

@@ -157,9 +157,9 @@ def test_chirp_length_rule():
     assert chirp_length("f32", 1001) == 2048
     assert chirp_length("f32", 700) == 1536
     assert chirp_length("f32", 4099) == 16384     # one kernel beats a four-step 12288
-    assert chirp_length("f64", 4099) == 10240     # no one-kernel FP64 16384: shortest split
-    assert chirp_length("f32", 10007) == 20480
-    assert chirp_length("f64", 7000) == 15360
+    assert chirp_length("f64", 4099) == 16384
+    assert chirp_length("f32", 10007) == 20480    # no one-kernel length: shortest split
+    assert chirp_length("f64", 9000) == 20480
     assert chirp_length("f32", (1 << 23)) == 1 << 24
 
 

@@ -20,11 +20,11 @@ from tests.gpu_util import (chirp_length, long_split, rel_l2, tiled_lengths, to_
 
 pytestmark = pytest.mark.gpu
 
-CASES = [  # (prec, n, layout); rows of 8192 (and FP32 16384) run one kernel
+CASES = [  # (prec, n, layout); rows of 8192 and 16384 run one kernel
     ("f32", 32768, "row"), ("f32", 8192, "col"), ("f64", 32768, "row"),
     ("f32", 32768, "gen"), ("f64", 16384, "col"), ("f32", 6144, "row"),
     ("f64", 5120, "gen"), ("f32", 65536, "row"), ("f32", 12288, "col"),
-    ("f64", 10240, "row"), ("f32", 1 << 20, "row"), ("f64", 16384, "row"),
+    ("f64", 10240, "row"), ("f32", 1 << 20, "row"), ("f64", 24576, "row"),
 ]
 
 
@@ -96,7 +96,7 @@ def test_long_c2c_two_level_batch_and_roundtrip():
 def test_long_c2c_measure_tuned():
     """Measure planning tunes the two sub-plans; the result is unchanged."""
     import parafaft_amd as P
-    n, batch = 16384, 4
+    n, batch = 32768, 4
     x = O.uniform_complex((batch * n,), seed=5).astype(np.complex64)
     ref = x.astype(np.complex128).copy()
     O.c2c_batched(ref, n, batch, 1, n, -1)

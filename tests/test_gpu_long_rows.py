@@ -1,5 +1,5 @@
-"""Rows beyond 4096 points in one kernel: 8192 (FP32 and FP64) and 16384
-(FP32) in the row layout run register-resident Stockham kernels (twiddles
+"""Rows beyond 4096 points in one kernel: 8192 and 16384 (FP32 and FP64)
+in the row layout run register-resident Stockham kernels (twiddles
 from the global table, a one-component LDS exchange); batch-contiguous
 layouts of these lengths keep the four-step plan.  Every candidate kernel on
 contiguous, gapped, strided and two-level row geometries, both directions
@@ -13,7 +13,7 @@ from tests.gpu_util import rel_l2, to_dev, to_host, tol
 
 pytestmark = pytest.mark.gpu
 
-ROWS = [("f32", 8192), ("f64", 8192), ("f32", 16384)]
+ROWS = [("f32", 8192), ("f64", 8192), ("f32", 16384), ("f64", 16384)]
 GEOMS = {  # batch, stride, dist, outer, outer gap
     "contiguous": (5, 1, None, 1, 0),
     "gapped": (3, 1, 7, 2, 33),
@@ -86,11 +86,12 @@ def test_long_rows_plan_kinds(prec, n):
     assert rel_l2(to_host(d), ref) <= tol(prec, n)
 
 
-@pytest.mark.parametrize("prec,n", [("f32", 16384), ("f64", 16384), ("f32", 32768)])
+@pytest.mark.parametrize("prec,n", [("f32", 16384), ("f64", 16384), ("f32", 32768),
+                                    ("f64", 32768)])
 def test_long_real_rows_through_one_kernel_half(prec, n):
     """Even real lengths past 8192: FP32 16384 runs one R2C kernel; FP64
-    16384 and FP32 32768 run their half (8192, 16384) as a one-kernel C2C
-    row, then the R2C post-pass; C2R back."""
+    16384 and FP32 / FP64 32768 run their half (8192, 16384) as a one-kernel
+    C2C row, then the R2C post-pass; C2R back."""
     import parafaft_amd as P
     batch = 3
     h = n // 2 + 1
