@@ -115,6 +115,14 @@ int64_t long_min_length() {
   return s ? atoll(s) : 2;
 }
 
+// Four-step plans make two passes where both lengths have the fused
+// kernels; PFX_FOURSTEP_PASSES=4 (measurement) keeps the four-pass form
+// (tiled sub-plans, twiddle, transpose).
+bool fourstep_two_pass() {
+  const char *s = getenv("PFX_FOURSTEP_PASSES");
+  return !(s && atoi(s) == 4);
+}
+
 // Kernel-choice override for measurement: PFX_KERNEL_MATCH=<substring of the
 // radix description>, e.g. "wide:32, 32".
 const char *kernel_match_override() { return getenv("PFX_KERNEL_MATCH"); }
@@ -175,6 +183,13 @@ struct LongPlan {
   // the transpose
   pfx_plan_t *c2 = nullptr;
   void *ptw1 = nullptr;
+  // four-step, rows (any caller that is not batch-contiguous): both passes
+  // fused (pfx_fourstep.hip), `ptw1` the length-n2 pass's table, `ptw2` the
+  // length-n1 pass's; no sub-plans
+  bool rows2 = false;
+  void *ptw2 = nullptr;
+  void *tw2 = nullptr;  // rows2: W_n^(c k2) at 16 k2 + c
+
   void *scratch = nullptr;  // the mode's work rows (four-step: one outer
                             // slice's footprint of the caller's layout)
   void *tw = nullptr;       // four-step W_n^i, i < n; real halves W_n^k; chirp w
@@ -310,6 +325,8 @@ static void free_long(pfx_plan p) {
   pfx_plan_destroy(p->lp->c);
   pfx_plan_destroy(p->lp->c2);
   if (p->lp->ptw1) hipFree(p->lp->ptw1);
+  if (p->lp->ptw2) hipFree(p->lp->ptw2);
+  if (p->lp->tw2) hipFree(p->lp->tw2);
   if (p->lp->scratch) hipFree(p->lp->scratch);
   if (p->lp->tw) hipFree(p->lp->tw);
   if (p->lp->filt) hipFree(p->lp->filt);
@@ -373,9 +390,18 @@ static int make_long(pfx_plan p) {
       lp->mode = LONG_FOURSTEP;
       lp->n2 = n / lp->n1;
       const int64_t n1 = lp->n1, n2 = lp->n2;
-      Radices rad1;
-      if (bc && (p->precision == PFX_F32 || p->precision == PFX_F64) &&
-          fourstep_cols_pass1_radices(n2, &rad1)) {
+      Radices rad1, radn1;
+      const bool plain = p->precision == PFX_F32 || p->precision == PFX_F64;
+      if (!bc && plain && fourstep_two_pass() && fourstep_rows_radices(n2, &rad1) &&
+          fourstep_rows_radices(n1, &radn1)) {
+        // two passes: the length-n2 DFTs with the twiddle into the scratch,
+        // then the length-n1 DFTs stored transposed into the caller's array
+        lp->rows2 = true;
+        PFX_HIP(hipMalloc(&lp->ptw1, (size_t)n2 * esz), "hipMalloc four-step pass table");
+        PFX_HIP(make_pass_twiddles(bp, rad1, lp->ptw1, nullptr), "four-step pass table");
+        PFX_HIP(hipMalloc(&lp->ptw2, (size_t)n1 * esz), "hipMalloc four-step pass table");
+        PFX_HIP(make_pass_twiddles(bp, radn1, lp->ptw2, nullptr), "four-step pass table");
+      } else if (bc && plain && fourstep_two_pass() && fourstep_cols_pass1_radices(n2, &rad1)) {
         // two passes: the fused first pass, then the length-n1 DFTs from
         // the scratch ((n2 j1 + k2) st + b) into the caller's array
         rc = plan_create(&lp->c2, p->precision, PFX_C2C, n1, B, n2 * st, 1, n2, st);
@@ -398,6 +424,11 @@ static int make_long(pfx_plan p) {
       }
       PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc long-plan twiddles");
       PFX_HIP(make_twiddles(bp, n, lp->tw, nullptr), "long-plan twiddles");
+      if (lp->rows2) {
+        PFX_HIP(hipMalloc(&lp->tw2, (size_t)(16 * n2) * esz), "hipMalloc four-step row table");
+        PFX_HIP(make_fourstep_rows_table(bp, lp->tw2, lp->tw, n2, nullptr),
+                "four-step row table");
+      }
     } else {
       if (n > kChirpMaxLength)
         return fail(PFX_ERR_UNSUPPORTED, "no split into two tiled lengths, and beyond the "
@@ -698,6 +729,29 @@ static int run_long(pfx_plan p, int direction, void *data, double scale, hipStre
     fa.backward = direction == PFX_BACKWARD;
     for (int64_t o = 0; o < p->outer; ++o) {
       char *base = static_cast<char *>(data) + (size_t)(o * p->dist_outer) * esz;
+      if (lp->rows2) {
+        FsRowArgs ra;
+        ra.src = base;
+        ra.dst = lp->scratch;
+        ra.ptw = lp->ptw1;
+        ra.tw = lp->tw;
+        ra.twc = lp->tw2;
+        ra.n1 = lp->n1;
+        ra.n2 = lp->n2;
+        ra.st = p->stride;
+        ra.dist = p->dist;
+        ra.rows = p->inner;
+        ra.tiles = 0;
+        ra.backward = direction == PFX_BACKWARD;
+        ra.scale = 1.0;
+        PFX_HIP(launch_fourstep_rows_pass(bp, 1, ra, s), "four-step rows first pass");
+        ra.src = lp->scratch;
+        ra.dst = base;
+        ra.ptw = lp->ptw2;
+        ra.scale = scale;
+        PFX_HIP(launch_fourstep_rows_pass(bp, 2, ra, s), "four-step rows second pass");
+        continue;
+      }
       if (lp->c2) {
         FsColArgs ca;
         ca.src = base;
@@ -932,6 +986,7 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
     // the sub-plans: `a` in place on the caller's scratch (its footprint is
     // the plan's), `c` on the plan's own scratch (C2C) or the caller's rows
     if (!scratch) return fail(PFX_ERR_INVALID_ARG, "tuning needs a scratch buffer");
+    if (p->lp->rows2) return PFX_SUCCESS;  // two fused passes: nothing to choose
     if (p->lp->a) {
       int rc = pfx_plan_tune(p->lp->a, flag, scratch, stream);
       if (rc != PFX_SUCCESS) return rc;

@@ -315,13 +315,14 @@ __global__ void __launch_bounds__(256) herm_full_store(const V2<T>* scr, T* rows
 // the plan makes two passes instead of four.  A tile is C consecutive
 // transforms b of one j1: its loads and stores run along the contiguous
 // batch axis.  The passes run in LDS (Stockham, natural order in and out).
-template <typename T, int N, int TPT, int C, int Ns, int... Rs> struct FsPasses;
-template <typename T, int N, int TPT, int C, int Ns> struct FsPasses<T, N, TPT, C, Ns> {
+template <typename T, int N, int TPT, int C, bool COL, int Ns, int... Rs> struct FsPasses;
+template <typename T, int N, int TPT, int C, bool COL, int Ns>
+struct FsPasses<T, N, TPT, C, COL, Ns> {
   __device__ static __forceinline__ void run(int, int, V2<T>*, const V2<T>*) {}
 };
-template <typename T, int N, int TPT, int C, int Ns, int R, int... Rest>
-struct FsPasses<T, N, TPT, C, Ns, R, Rest...> {
-  using L = LdsLayout<T, N, C, true>;
+template <typename T, int N, int TPT, int C, bool COL, int Ns, int R, int... Rest>
+struct FsPasses<T, N, TPT, C, COL, Ns, R, Rest...> {
+  using L = LdsLayout<T, N, C, COL>;
   struct StoreLds {
     V2<T>* lds;
     int c;
@@ -332,7 +333,35 @@ struct FsPasses<T, N, TPT, C, Ns, R, Rest...> {
   __device__ static __forceinline__ void run(int j, int c, V2<T>* lds, const V2<T>* stw) {
     auto load = [&](int e) -> V2<T> { return lds[L::at(e, c)]; };
     stockham_pass<T, N, R, Ns, TPT>(j, stw, load, StoreLds{lds, c});
-    FsPasses<T, N, TPT, C, Ns * R, Rest...>::run(j, c, lds, stw);
+    FsPasses<T, N, TPT, C, COL, Ns * R, Rest...>::run(j, c, lds, stw);
+  }
+};
+
+// The passes of one tile with the first pass loading from HBM through `ld`
+// and the last storing to HBM through `st` (element and output indices of
+// the transform), the passes between them in LDS.
+template <typename T, int N, int TPT, int C, bool COL, int Ns, int... Rs> struct FsRun;
+template <typename T, int N, int TPT, int C, bool COL, int Ns>
+struct FsRun<T, N, TPT, C, COL, Ns> {
+  template <class LD, class ST>
+  __device__ static __forceinline__ void run(int, int, V2<T>*, const V2<T>*, const LD&,
+                                             const ST&) {}
+};
+template <typename T, int N, int TPT, int C, bool COL, int Ns, int R, int... Rest>
+struct FsRun<T, N, TPT, C, COL, Ns, R, Rest...> {
+  using P = FsPasses<T, N, TPT, C, COL, Ns, R, Rest...>;
+  template <class LD, class ST>
+  __device__ static __forceinline__ void run(int j, int c, V2<T>* lds, const V2<T>* stw,
+                                             const LD& ld, const ST& st) {
+    using L = LdsLayout<T, N, C, COL>;
+    auto from_lds = [&](int e) -> V2<T> { return lds[L::at(e, c)]; };
+    typename P::StoreLds to_lds{lds, c};
+    constexpr bool first = Ns == 1, last = sizeof...(Rest) == 0;
+    if constexpr (first && last) stockham_pass<T, N, R, Ns, TPT>(j, stw, ld, st);
+    else if constexpr (first) stockham_pass<T, N, R, Ns, TPT>(j, stw, ld, to_lds);
+    else if constexpr (last) stockham_pass<T, N, R, Ns, TPT>(j, stw, from_lds, st);
+    else stockham_pass<T, N, R, Ns, TPT>(j, stw, from_lds, to_lds);
+    FsRun<T, N, TPT, C, COL, Ns * R, Rest...>::run(j, c, lds, stw, ld, st);
   }
 };
 
@@ -354,26 +383,28 @@ __global__ void __launch_bounds__(TPT * C) fourstep_cols_pass1(FsColArgs a) {
   const bool bw = a.backward != 0;
   const V2<T>* src = static_cast<const V2<T>*>(a.src) + (j1 * a.st + (valid ? b : 0));
   const int64_t ls = a.n1 * a.st;
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int e = j + m * TPT;
+  auto ld = [&](int e) -> V2<T> {
     V2<T> x = mk<T>(T(0), T(0));
     if (valid) x = src[e * ls];
-    lds[L::at(e, c)] = bw ? swp<T>(x) : x;
-  }
-  __syncthreads();
-  FsPasses<T, N, TPT, C, 1, Rs...>::run(j, c, lds, stw);
+    return bw ? swp<T>(x) : x;
+  };
+  struct Store {
+    V2<T>* dst;
+    const V2<T>* w;
+    int64_t st, j1;
+    bool bw, valid;
+    __device__ void before() const {}
+    __device__ void after() const {}
+    __device__ void operator()(int k2, V2<T> y) const {
+      if (bw) y = swp<T>(y);
+      const V2<T> t = w[j1 * k2];  // j1 k2 <= (n1 - 1)(n2 - 1) < n
+      y = bw ? cmulc<T>(y, t) : cmul<T>(y, t);
+      if (valid) dst[k2 * st] = y;
+    }
+  };
   V2<T>* dst = static_cast<V2<T>*>(a.dst) + ((int64_t)N * j1 * a.st + (valid ? b : 0));
-  const V2<T>* w = static_cast<const V2<T>*>(a.tw);
-#pragma unroll
-  for (int m = 0; m < E; ++m) {
-    const int k2 = j + m * TPT;
-    V2<T> y = lds[L::at(k2, c)];
-    if (bw) y = swp<T>(y);
-    const V2<T> t = w[j1 * k2];  // j1 k2 <= (n1 - 1)(n2 - 1) < n
-    y = bw ? cmulc<T>(y, t) : cmul<T>(y, t);
-    if (valid) dst[k2 * a.st] = y;
-  }
+  FsRun<T, N, TPT, C, true, 1, Rs...>::run(
+      j, c, lds, stw, ld, Store{dst, static_cast<const V2<T>*>(a.tw), a.st, j1, bw, valid});
 }
 
 template <typename T, int TPT, int... Rs>
@@ -389,12 +420,171 @@ void launch_cols_pass1(const FsColArgs& a, hipStream_t s) {
   fourstep_cols_pass1<T, TPT, C, Rs...><<<dim3((unsigned)blocks), dim3(TPT * C), 0, s>>>(b);
 }
 
+// Columns per entry of the first row pass's table W_n^(c k2) (tiles hold at
+// most 16 adjacent j1).
+constexpr int kRowsTabC = 16;
+
+// Two-pass four-step for rows (element j1 + n1 j2 of row r at r dist +
+// (j1 + n1 j2) st).  Pass 1: the length-n2 DFTs of C adjacent j1 of one row
+// in LDS (lanes across j1, so a load of one j2 is C consecutive elements),
+// each output times W_n^(j1 k2), stored to the scratch at the same offsets
+// (j1 + n1 k2).  Pass 2: the length-n1 DFTs of C adjacent k2 from the
+// scratch (one contiguous run of C n1 elements per tile, loaded linearly),
+// stored transposed to n2 k1 + k2 of the caller's array (lanes across k2:
+// runs of C elements) with the caller's scale.
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) fourstep_rows_pass1(FsRowArgs a) {
+  constexpr int N = (Rs * ... * 1);  // n2
+  constexpr int E = N / TPT;
+  static_assert(N % TPT == 0, "whole elements per thread");
+  static_assert(C <= kRowsTabC && kRowsTabC % C == 0, "the twiddle table's columns");
+  using L = LdsLayout<T, N, C, true>;
+  using B = LdsBlock<T, V2<T>, N, L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<T>* lds = B::data(smem);
+  const V2<T>* stw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(B::tw(smem), a.ptw);
+  int j, c;
+  thread_coords<TPT, C, true>(j, c);
+  // workgroup order as dispatched (adjacent tiles of a row on different
+  // XCDs at the same time): the XCD-contiguous order measured 4% faster at
+  // 2^15 points and 15% slower at 2^20
+  const int64_t r = blockIdx.x / a.tiles;
+  const int64_t j1 = (blockIdx.x - r * a.tiles) * C + c;
+  const bool valid = j1 < a.n1;
+  const bool bw = a.backward != 0;
+  const int64_t off = r * a.dist + (valid ? j1 : 0) * a.st;
+  const V2<T>* src = static_cast<const V2<T>*>(a.src) + off;
+  V2<T>* dst = static_cast<V2<T>*>(a.dst) + off;
+  const int64_t ls = a.n1 * a.st;
+  // W_n^(j1 k2) = W_n^(c k2) W_n^(j10 k2), j1 = j10 + c: the first from a
+  // table laid out [k2][16] (the lanes' c are consecutive, so one load is
+  // one run), the second the same for the 16 lanes of one k2 (the lanes' j1
+  // differ, so W_n^(j1 k2) from the table of n would scatter their loads)
+  const V2<T>* wn = static_cast<const V2<T>*>(a.tw);
+  const V2<T>* wc = static_cast<const V2<T>*>(a.twc);
+  const int64_t j10 = j1 - c;
+  auto ld = [&](int e) -> V2<T> {
+    V2<T> x = mk<T>(T(0), T(0));
+    if (valid) x = src[e * ls];
+    return bw ? swp<T>(x) : x;
+  };
+  struct Store {
+    V2<T>* dst;
+    const V2<T>* wn;
+    const V2<T>* wc;
+    int64_t ls, j10;
+    int c;
+    bool bw, valid;
+    __device__ void before() const {}
+    __device__ void after() const {}
+    __device__ void operator()(int k2, V2<T> y) const {
+      if (bw) y = swp<T>(y);
+      // j10 k2 <= (n1 - C)(n2 - 1) < n, c k2 < 16 n2 <= n
+      const V2<T> t = cmul<T>(wc[k2 * kRowsTabC + c], wn[j10 * k2]);
+      y = bw ? cmulc<T>(y, t) : cmul<T>(y, t);
+      if (valid) dst[k2 * ls] = y;
+    }
+  };
+  FsRun<T, N, TPT, C, true, 1, Rs...>::run(j, c, lds, stw, ld,
+                                           Store{dst, wn, wc, ls, j10, c, bw, valid});
+}
+
+template <typename T, int TPT, int C, int... Rs>
+__global__ void __launch_bounds__(TPT * C) fourstep_rows_pass2(FsRowArgs a) {
+  constexpr int N = (Rs * ... * 1);  // n1
+  constexpr int E = N / TPT;
+  static_assert(N % TPT == 0, "whole elements per thread");
+  using L = LdsLayout<T, N, C, false>;
+  using B = LdsBlock<T, V2<T>, N, L::size>;
+  __shared__ __align__(16) unsigned char smem[B::bytes];
+  V2<T>* lds = B::data(smem);
+  const V2<T>* stw = stage_twiddles<T, N, sizeof...(Rs), TPT * C>(B::tw(smem), a.ptw);
+  const int tid = lane_tid();
+  const int64_t r = blockIdx.x / a.tiles;
+  const int64_t k20 = (blockIdx.x - r * a.tiles) * C;
+  const bool bw = a.backward != 0;
+  int j, c;
+  thread_coords<TPT, C, false>(j, c);
+  // element e of the tile's transform c sits at (n1 (k20 + c) + e) st
+  const bool valid = k20 + c < a.n2;
+  const V2<T>* src = static_cast<const V2<T>*>(a.src) + r * a.dist +
+                     a.n1 * (valid ? k20 + c : 0) * a.st;
+  auto ld = [&](int e) -> V2<T> {
+    V2<T> x = mk<T>(T(0), T(0));
+    if (valid) x = src[e * a.st];
+    return bw ? swp<T>(x) : x;
+  };
+  // the outputs stay in LDS: the stores below run across the tile's
+  // transforms (adjacent k2) for one k1
+  typename FsPasses<T, N, TPT, C, false, 1, Rs...>::StoreLds to_lds{lds, c};
+  FsRun<T, N, TPT, C, false, 1, Rs...>::run(j, c, lds, stw, ld, to_lds);
+  V2<T>* dst = static_cast<V2<T>*>(a.dst) + r * a.dist;
+  const bool use_scale = a.scale != 1.0;
+  const T sc = T(a.scale);
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int i = tid + m * TPT * C;
+    const int t = i % C, k1 = i / C;
+    V2<T> y = lds[L::at(k1, t)];
+    if (bw) y = swp<T>(y);
+    if (use_scale) y = scl<T>(y, sc);
+    if (k20 + t < a.n2) dst[(a.n2 * k1 + k20 + t) * a.st] = y;
+  }
+}
+
+// Tiles: pass 1 as the batch-contiguous first pass (16 columns up to 512
+// points, 8-16 beyond with the CU's whole LDS); pass 2 16 transforms per
+// tile (128-byte runs of the transposed stores), halved while the padded
+// rows exceed the LDS or the tile 1024 threads.
+template <int N, int TPT, int ESZ> constexpr int rows_pass2_tile() {
+  int c = 16;
+  while (c > 1 && (c * (N + N / 16 + 1) * ESZ + N * ESZ > kLdsBig || TPT * c > 1024)) c /= 2;
+  return c;
+}
+
+template <typename T, int TPT, int... Rs>
+void launch_rows_pass1(const FsRowArgs& a, hipStream_t s) {
+  constexpr int N = (Rs * ... * 1);
+  constexpr int C = col_tile<N, TPT, (int)sizeof(V2<T>), (N >= 1024 ? kLdsBig : kLdsBudget)>();
+  FsRowArgs b = a;
+  b.tiles = (a.n1 + C - 1) / C;
+  fourstep_rows_pass1<T, TPT, C, Rs...><<<dim3((unsigned)(a.rows * b.tiles)), dim3(TPT * C), 0, s>>>(b);
+}
+
+template <typename T, int TPT, int... Rs>
+void launch_rows_pass2(const FsRowArgs& a, hipStream_t s) {
+  constexpr int N = (Rs * ... * 1);
+  constexpr int C = rows_pass2_tile<N, TPT, (int)sizeof(V2<T>)>();
+  FsRowArgs b = a;
+  b.tiles = (a.n2 + C - 1) / C;
+  fourstep_rows_pass2<T, TPT, C, Rs...><<<dim3((unsigned)(a.rows * b.tiles)), dim3(TPT * C), 0, s>>>(b);
+}
+
 int grid_for(int64_t work) {
   const int64_t blocks = (work + 255) / 256;
   return (int)(blocks < 65536 ? (blocks > 0 ? blocks : 1) : 65536);
 }
 
 }  // namespace
+
+template <typename T>
+__global__ void __launch_bounds__(256) rows_pass1_table_kernel(V2<T>* out, const V2<T>* tw,
+                                                               int64_t n2) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n2 * kRowsTabC) out[i] = tw[(i % kRowsTabC) * (i / kRowsTabC)];
+}
+
+hipError_t make_fourstep_rows_table(int prec, void* out, const void* tw, int64_t n2,
+                                    hipStream_t s) {
+  const int g = (int)((n2 * kRowsTabC + 255) / 256);
+  if (prec)
+    rows_pass1_table_kernel<double><<<g, 256, 0, s>>>(static_cast<V2<double>*>(out),
+                                                      static_cast<const V2<double>*>(tw), n2);
+  else
+    rows_pass1_table_kernel<float><<<g, 256, 0, s>>>(static_cast<V2<float>*>(out),
+                                                     static_cast<const V2<float>*>(tw), n2);
+  return hipGetLastError();
+}
 
 hipError_t launch_fourstep_twiddle(int prec, void* buf, const void* tw, const FourStepArgs& a,
                                    hipStream_t s) {
@@ -559,6 +749,45 @@ const ColsPass1* cols_pass1(int64_t n2) {
   return nullptr;
 }
 }  // namespace
+
+namespace {
+struct RowsPass {
+  int n;
+  Radices rad;
+  void (*p1f32)(const FsRowArgs&, hipStream_t);
+  void (*p1f64)(const FsRowArgs&, hipStream_t);
+  void (*p2f32)(const FsRowArgs&, hipStream_t);
+  void (*p2f64)(const FsRowArgs&, hipStream_t);
+};
+#define PFX_FS_ROWS(TPT, ...)                                                               \
+  {prod<__VA_ARGS__>(), radices<__VA_ARGS__>(), &launch_rows_pass1<float, TPT, __VA_ARGS__>, \
+   &launch_rows_pass1<double, TPT, __VA_ARGS__>, &launch_rows_pass2<float, TPT, __VA_ARGS__>, \
+   &launch_rows_pass2<double, TPT, __VA_ARGS__>},
+const RowsPass kRowsPass[] = {
+    PFX_FS_ROWS(8, 8, 8) PFX_FS_ROWS(8, 16, 8) PFX_FS_ROWS(16, 16, 16)
+    PFX_FS_ROWS(64, 8, 8, 8) PFX_FS_ROWS(64, 16, 16, 4) PFX_FS_ROWS(128, 16, 16, 8)};
+#undef PFX_FS_ROWS
+const RowsPass* rows_pass(int64_t n) {
+  for (const RowsPass& e : kRowsPass)
+    if (e.n == n) return &e;
+  return nullptr;
+}
+}  // namespace
+
+bool fourstep_rows_radices(int64_t n, Radices* rad) {
+  const RowsPass* e = rows_pass(n);
+  if (e && rad) *rad = e->rad;
+  return e != nullptr;
+}
+
+hipError_t launch_fourstep_rows_pass(int prec, int pass, const FsRowArgs& a, hipStream_t s) {
+  const RowsPass* e = rows_pass(pass == 1 ? a.n2 : a.n1);
+  if (!e || (pass != 1 && pass != 2)) return hipErrorInvalidValue;
+  if (a.rows == 0) return hipSuccess;
+  if (pass == 1) (prec ? e->p1f64 : e->p1f32)(a, s);
+  else (prec ? e->p2f64 : e->p2f32)(a, s);
+  return hipGetLastError();
+}
 
 bool fourstep_cols_pass1_radices(int64_t n2, Radices* rad) {
   const ColsPass1* e = cols_pass1(n2);

@@ -162,6 +162,26 @@ struct FsColArgs {
 // The radices of the fused first pass for n2; false: none for this n2.
 bool fourstep_cols_pass1_radices(int64_t n2, Radices *rad);
 hipError_t launch_fourstep_cols_pass1(int prec, const FsColArgs &a, hipStream_t s);
+// Two-pass four-step of rows (row r at r dist, element stride st): pass 1
+// the length-n2 DFTs times W_n^(j1 k2) from the caller's array into the
+// scratch at the same offsets, pass 2 the length-n1 DFTs from the scratch,
+// stored transposed into the caller's array with the scale.
+struct FsRowArgs {
+  const void *src;  // pass 1: the caller's array; pass 2: the scratch
+  void *dst;        // pass 1: the scratch; pass 2: the caller's array
+  const void *ptw;  // the pass's per-pass twiddle table (n2, resp. n1)
+  const void *tw;   // pass 1: W_n^m, m < n
+  const void *twc;  // pass 1: W_n^(c k2) at 16 k2 + c, c < 16, k2 < n2
+  int64_t n1, n2, st, dist, rows;
+  int64_t tiles;    // tiles per row, set by the launcher
+  int backward;
+  double scale;     // pass 2
+};
+// The radices of either pass for length n; false: none (four passes).
+bool fourstep_rows_radices(int64_t n, Radices *rad);
+// The first row pass's table W_n^(c k2) (16 n2 entries) from the table of n.
+hipError_t make_fourstep_rows_table(int prec, void *out, const void *tw, int64_t n2, hipStream_t s);
+hipError_t launch_fourstep_rows_pass(int prec, int pass, const FsRowArgs &a, hipStream_t s);
 // Long R2C post-pass / C2R pre-pass on padded rows (row b at b * dist complex
 // slots, h + 1 slots each); tw: W_2h^k, k <= h.
 struct RealPassArgs {

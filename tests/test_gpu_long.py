@@ -406,13 +406,17 @@ def test_chirp_and_odd_real_batches_run_in_slices():
 @pytest.mark.parametrize("prec", ["f32", "f64"])
 @pytest.mark.parametrize("layout", ["row", "col"])
 @pytest.mark.parametrize("n", [8192, 1 << 20])
-def test_long_every_sub_plan_kernel(prec, layout, n, tmp_path):
+def test_long_every_sub_plan_kernel(prec, layout, n, tmp_path, monkeypatch):
     """Every kernel candidate of both tiled sub-plans of a four-step plan
     (8192 = 64 x 128 in columns, 32768 = 128 x 256 in rows, 2^20 = 1024 x
     1024), forced in turn through a tuning-cache line: the first pass runs
     out of place into the plan's scratch, so every kernel must load from the
-    caller's array; both directions against the oracle, gaps untouched."""
+    caller's array; both directions against the oracle, gaps untouched.
+    Rows take the four-pass form here (their default two-pass form has no
+    sub-plans)."""
     from tests.gpu_util import forced_sub_plan_kernels
+    if layout == "row":
+        monkeypatch.setenv("PFX_FOURSTEP_PASSES", "4")
     if n == 8192 and layout == "row":
         n = 32768  # rows of 8192 run one kernel
     batch, stride, dist = _geometry(n, layout)
