@@ -545,7 +545,8 @@ template <int N, int TPT, int ESZ> constexpr int rows_pass2_tile() {
 template <typename T, int TPT, int... Rs>
 void launch_rows_pass1(const FsRowArgs& a, hipStream_t s) {
   constexpr int N = (Rs * ... * 1);
-  constexpr int C = col_tile<N, TPT, (int)sizeof(V2<T>), (N >= 1024 ? kLdsBig : kLdsBudget)>();
+  constexpr int C0 = col_tile<N, TPT, (int)sizeof(V2<T>), (N >= 1024 ? kLdsBig : kLdsBudget)>();
+  constexpr int C = C0 < kRowsTabC ? C0 : kRowsTabC;
   FsRowArgs b = a;
   b.tiles = (a.n1 + C - 1) / C;
   fourstep_rows_pass1<T, TPT, C, Rs...><<<dim3((unsigned)(a.rows * b.tiles)), dim3(TPT * C), 0, s>>>(b);
@@ -741,7 +742,14 @@ struct ColsPass1 {
    &launch_cols_pass1<double, TPT, __VA_ARGS__>},
 const ColsPass1 kColsPass1[] = {
     PFX_FS_COLS(8, 8, 8) PFX_FS_COLS(8, 16, 8) PFX_FS_COLS(16, 16, 16)
-    PFX_FS_COLS(64, 8, 8, 8) PFX_FS_COLS(64, 16, 16, 4) PFX_FS_COLS(128, 16, 16, 8)};
+    PFX_FS_COLS(64, 8, 8, 8) PFX_FS_COLS(64, 16, 16, 4) PFX_FS_COLS(128, 16, 16, 8)
+    // smooth factors (the tiled plans' radices)
+    PFX_FS_COLS(5, 16, 5) PFX_FS_COLS(12, 8, 4, 3) PFX_FS_COLS(15, 8, 3, 5)
+    PFX_FS_COLS(20, 8, 4, 5) PFX_FS_COLS(12, 16, 4, 3) PFX_FS_COLS(15, 16, 3, 5)
+    PFX_FS_COLS(20, 16, 4, 5) PFX_FS_COLS(24, 16, 8, 3) PFX_FS_COLS(40, 16, 8, 5)
+    PFX_FS_COLS(48, 16, 16, 3) PFX_FS_COLS(80, 16, 16, 5) PFX_FS_COLS(48, 32, 16, 3)
+    PFX_FS_COLS(128, 32, 16, 5) PFX_FS_COLS(192, 16, 16, 4, 3) PFX_FS_COLS(20, 5, 5, 4)
+    PFX_FS_COLS(125, 8, 5, 5, 5)};
 #undef PFX_FS_COLS
 const ColsPass1* cols_pass1(int64_t n2) {
   for (const ColsPass1& e : kColsPass1)
@@ -765,7 +773,14 @@ struct RowsPass {
    &launch_rows_pass2<double, TPT, __VA_ARGS__>},
 const RowsPass kRowsPass[] = {
     PFX_FS_ROWS(8, 8, 8) PFX_FS_ROWS(8, 16, 8) PFX_FS_ROWS(16, 16, 16)
-    PFX_FS_ROWS(64, 8, 8, 8) PFX_FS_ROWS(64, 16, 16, 4) PFX_FS_ROWS(128, 16, 16, 8)};
+    PFX_FS_ROWS(64, 8, 8, 8) PFX_FS_ROWS(64, 16, 16, 4) PFX_FS_ROWS(128, 16, 16, 8)
+    // smooth factors (the tiled plans' radices)
+    PFX_FS_ROWS(5, 16, 5) PFX_FS_ROWS(12, 8, 4, 3) PFX_FS_ROWS(15, 8, 3, 5)
+    PFX_FS_ROWS(20, 8, 4, 5) PFX_FS_ROWS(12, 16, 4, 3) PFX_FS_ROWS(15, 16, 3, 5)
+    PFX_FS_ROWS(20, 16, 4, 5) PFX_FS_ROWS(24, 16, 8, 3) PFX_FS_ROWS(40, 16, 8, 5)
+    PFX_FS_ROWS(48, 16, 16, 3) PFX_FS_ROWS(80, 16, 16, 5) PFX_FS_ROWS(48, 32, 16, 3)
+    PFX_FS_ROWS(128, 32, 16, 5) PFX_FS_ROWS(192, 16, 16, 4, 3) PFX_FS_ROWS(20, 5, 5, 4)
+    PFX_FS_ROWS(125, 8, 5, 5, 5)};
 #undef PFX_FS_ROWS
 const RowsPass* rows_pass(int64_t n) {
   for (const RowsPass& e : kRowsPass)
