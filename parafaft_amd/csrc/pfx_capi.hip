@@ -123,6 +123,13 @@ bool fourstep_two_pass() {
   return !(s && atoi(s) == 4);
 }
 
+// Chirp plans whose convolution length has a tiled radix plan run one fused
+// kernel; PFX_CHIRP_FUSED=0 (measurement) keeps the five-pass form.
+bool chirp_fused_enabled() {
+  const char *s = getenv("PFX_CHIRP_FUSED");
+  return !(s && atoi(s) == 0);
+}
+
 // Kernel-choice override for measurement: PFX_KERNEL_MATCH=<substring of the
 // radix description>, e.g. "wide:32, 32".
 const char *kernel_match_override() { return getenv("PFX_KERNEL_MATCH"); }
@@ -186,6 +193,7 @@ struct LongPlan {
   // four-step, rows (any caller that is not batch-contiguous): both passes
   // fused (pfx_fourstep.hip), `ptw1` the length-n2 pass's table, `ptw2` the
   // length-n1 pass's; no sub-plans
+  const ChirpFused *cf = nullptr;  // LONG_CHIRP in one kernel (`ptw1` its pass table)
   bool rows2 = false;
   void *ptw2 = nullptr;
   void *tw2 = nullptr;  // rows2: W_n^(c k2) at 16 k2 + c
@@ -436,11 +444,21 @@ static int make_long(pfx_plan p) {
                     (long long)kChirpMaxLength);
       lp->mode = LONG_CHIRP;
       lp->M = chirp_length(bp, n);
-      lp->rows = std::max<int64_t>(1, std::min<int64_t>(total, kLongRowsBytes / (lp->M * (int64_t)esz)));
-      rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->M, lp->rows, 1, lp->M, 1, 0);
-      if (rc != PFX_SUCCESS) return rc;
-      if (total > 0)
-        PFX_HIP(hipMalloc(&lp->scratch, (size_t)(lp->rows * lp->M) * esz), "hipMalloc chirp rows");
+      if ((p->precision == PFX_F32 || p->precision == PFX_F64) && chirp_fused_enabled())
+        lp->cf = bp ? chirp_fused_f64(lp->M) : chirp_fused_f32(lp->M);
+      if (lp->cf) {
+        // one kernel per execution, in place: no scratch rows, no sub-plan
+        PFX_HIP(hipMalloc(&lp->ptw1, (size_t)lp->M * esz), "hipMalloc chirp pass table");
+        PFX_HIP(make_pass_twiddles(bp, lp->cf->rad, lp->ptw1, nullptr), "chirp pass table");
+      } else {
+        lp->rows = std::max<int64_t>(
+            1, std::min<int64_t>(total, kLongRowsBytes / (lp->M * (int64_t)esz)));
+        rc = plan_create(&lp->c, p->precision, PFX_C2C, lp->M, lp->rows, 1, lp->M, 1, 0);
+        if (rc != PFX_SUCCESS) return rc;
+        if (total > 0)
+          PFX_HIP(hipMalloc(&lp->scratch, (size_t)(lp->rows * lp->M) * esz),
+                  "hipMalloc chirp rows");
+      }
       PFX_HIP(hipMalloc(&lp->tw, (size_t)n * esz), "hipMalloc chirp");
       PFX_HIP(hipMalloc(&lp->filt, (size_t)(2 * lp->M) * esz), "hipMalloc chirp filters");
       PFX_HIP(make_chirp(bp, lp->tw, n, nullptr), "chirp");
@@ -677,6 +695,25 @@ static int run_long(pfx_plan p, int direction, void *data, double scale, hipStre
   // The chirp and odd-real modes run the batch in slices of lp->rows
   // transforms; the sub-plan always transforms all scratch rows (on the
   // last, short slice the rows past it are stale and unused).
+  if (lp->mode == LONG_CHIRP && lp->cf) {
+    ChirpArgs ca;
+    ca.n = p->length;
+    ca.M = lp->M;
+    ca.t0 = 0;
+    ca.total = p->inner * p->outer;
+    ca.inner = p->inner;
+    ca.dist = p->dist;
+    ca.dist_outer = p->dist_outer;
+    ca.stride = p->stride;
+    ca.backward = direction == PFX_BACKWARD;
+    ca.scale = scale;
+    if (ca.total == 0) return PFX_SUCCESS;
+    // batch-contiguous tiles where neighbouring transforms are adjacent
+    const bool col = p->dist == 1 && p->stride > 1;
+    (col ? lp->cf->col : lp->cf->row)(data, lp->tw, lp->filt, lp->ptw1, ca, s);
+    PFX_HIP(hipGetLastError(), "chirp kernel");
+    return PFX_SUCCESS;
+  }
   if (lp->mode == LONG_CHIRP) {
     const int64_t total = p->inner * p->outer;
     ChirpArgs ca;
@@ -986,7 +1023,7 @@ int pfx_plan_tune(pfx_plan p, int flag, void *scratch, pfx_stream stream) {
     // the sub-plans: `a` in place on the caller's scratch (its footprint is
     // the plan's), `c` on the plan's own scratch (C2C) or the caller's rows
     if (!scratch) return fail(PFX_ERR_INVALID_ARG, "tuning needs a scratch buffer");
-    if (p->lp->rows2) return PFX_SUCCESS;  // two fused passes: nothing to choose
+    if (p->lp->rows2 || p->lp->cf) return PFX_SUCCESS;  // fused kernels: nothing to choose
     if (p->lp->a) {
       int rc = pfx_plan_tune(p->lp->a, flag, scratch, stream);
       if (rc != PFX_SUCCESS) return rc;

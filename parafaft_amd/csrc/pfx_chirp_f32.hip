@@ -1,0 +1,3 @@
+#define PFX_T float
+#define PFX_CHIRP_FN chirp_fused_f32
+#include "pfx_chirp.inc"

@@ -1,0 +1,3 @@
+#define PFX_T double
+#define PFX_CHIRP_FN chirp_fused_f64
+#include "pfx_chirp.inc"

@@ -212,6 +212,20 @@ hipError_t launch_chirp_mul(int prec, void *scr, const void *B, const ChirpArgs 
                             hipStream_t s);
 hipError_t launch_chirp_post(int prec, const void *scr, void *x, const void *w,
                              const ChirpArgs &a, hipStream_t s);
+// The chirp convolution in one kernel for a convolution length M with a
+// tiled radix plan (pfx_chirp.inc): x in place, w the chirp (n), B the
+// forward filter's spectrum (M), ptw the per-pass table of `rad`; t0 = 0,
+// total = every transform of the plan; row: transforms along rows
+// (row-layout tiles), col: batch-contiguous tiles.
+using ChirpFusedFn = void (*)(void *x, const void *w, const void *B, const void *ptw,
+                              const ChirpArgs &a, hipStream_t s);
+struct ChirpFused {
+  int M;
+  Radices rad;
+  ChirpFusedFn row, col;
+};
+const ChirpFused *chirp_fused_f32(int64_t M);
+const ChirpFused *chirp_fused_f64(int64_t M);
 // Odd real lengths through a length-n C2C on scratch rows of n: which = 0
 // R2C load (reals -> complex), 1 R2C store (X[0..n/2] -> padded row), 2 C2R
 // load (Hermitian extension), 3 C2R store (real parts -> row).  Row b at
