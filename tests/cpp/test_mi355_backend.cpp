@@ -284,12 +284,12 @@ int main() {
   test_different_pointers<float>();
   test_r2c_c2r<float>();
   std::printf("lengths without a single-kernel plan\n");
-  for (int n : {100, 1000, 8191, 32768}) {
+  for (int n : {144, 1009, 8191, 32768, 100000}) {
     test_any_length<float>(n);
     test_any_length<double>(n);
   }
-  std::printf("rows beyond 4096 in one kernel (FP64 16384: four-step)\n");
-  for (int n : {8192, 16384}) {
+  std::printf("smooth lengths and rows beyond 4096 in one kernel\n");
+  for (int n : {100, 768, 1000, 3072, 8192, 16384}) {
     test_any_length<float>(n);
     test_any_length<double>(n);
   }
