@@ -1,5 +1,6 @@
 """The chirp convolution (Bluestein) in one kernel (pfx_chirp.inc) for
-lengths whose convolution length M has a tiled radix plan (n up to 2048):
+lengths whose convolution length M has a tiled radix plan (n up to 2048),
+and M = 8192 in FP32 (n up to 4096):
 premultiply, M-point FFT, filter spectrum, inverse FFT and postmultiply on
 one LDS tile.  Against the oracle in rows, batch-contiguous columns, strided
 rows with gaps and two-level batches, both directions with a fused scale,
@@ -14,7 +15,7 @@ from tests.gpu_util import chirp_length, long_split, rel_l2, to_dev, to_host, to
 
 pytestmark = pytest.mark.gpu
 
-LENGTHS = [7, 11, 13, 17, 31, 97, 127, 257, 509, 1009, 1021, 2039, 2047, 1001, 1100]
+LENGTHS = [7, 11, 13, 17, 31, 97, 127, 257, 509, 1009, 1021, 2039, 2047, 1001, 1100, 2053, 4093]
 GEOMS = {  # batch, stride, dist(n), outer, outer gap
     "rows": (5, 1, lambda n: n, 1, 0),
     "cols": (37, 37, lambda n: 1, 1, 0),
