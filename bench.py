@@ -506,6 +506,16 @@ def main():
         xr = torch.view_as_real(x.view(-1))
         xr.copy_(torch.rand(xr.shape, device=dev, generator=g, dtype=rdt) * 2 - 1)
         torch.cuda.synchronize()
+    # The timed steps below run the chunked schedule part by part from its
+    # stage plans; a split axis 0 (chosen by the planner only where it is
+    # faster, e.g. 2048^3) has no such parts, so it is turned off here and
+    # the line says so.
+    split_c = C.c_int()
+    A.check(L.pfx_ndplan_split(nd.value, C.byref(split_c)), "pfx_ndplan_split")
+    split_tuned = split_c.value
+    if split_tuned:
+        log(f"[bench] planner chose a split axis 0 (G = {split_tuned}); timing the chunked parts")
+        A.check(L.pfx_ndplan_set_split(nd.value, 0), "pfx_ndplan_set_split")
     stages = []
     for s in range(D):
         h = C.c_void_p()
@@ -712,6 +722,7 @@ def main():
                                   "one launch per axis)"),
                    "grid": [n, n, n], "precision": args.precision, "planning": args.plan,
                    "chunk_planes": chunk,
+                   "split_axis0_chosen_by_planner": split_tuned,
                    "stage_kernels": names,
                    "chunk_kernels": [describe(h) for h in cstages]},
         "hbm_tbs_algorithmic": round(tbs, 3),

@@ -347,6 +347,26 @@ int pfx_ndplan_chunk_stage(pfx_ndplan plan, int stage, pfx_plan *out);
 /* Stage plan of axis-stage s (0 = last axis), for per-stage timing. */
 int pfx_ndplan_stage(pfx_ndplan plan, int stage, pfx_plan *out);
 
+/*
+ * Split axis 0 (C2C FP32/FP64, chunked schedule): n0 = q G, with
+ * x0 = x1 + q x2 and k0 = k2 + G k1.  Per chunk, one kernel takes the
+ * G-point DFTs along axis 0 of planes x1 + q x2, the twiddle W_n0^(x1 k2)
+ * and the FFT along the last axis, and stores plane G x1 + k2 of a scratch
+ * array (so a chunk's planes stay consecutive); the middle axes run there;
+ * then the q-point DFTs along axis 0 (stride G planes) go from the scratch
+ * into the array.  Axis 0's column runs get q / n0 of the rows to cross
+ * instead of n0, at the cost of one array-sized scratch the plan holds.
+ * g = 0 turns it off; pfx_ndplan_tune (Measure/Patient) keeps it only if it
+ * times faster than the chunked schedule.  Changing the chunk turns it off.
+ * No counterpart in the reference (its stage loop is
+ * parafaft_c2c.hpp:539-602).
+ */
+int pfx_ndplan_split(pfx_ndplan plan, int *g);
+int pfx_ndplan_set_split(pfx_ndplan plan, int g);
+/* The q-point plan of the split schedule (applied from the scratch into the
+ * array, forward), for per-stage timing and kernel names. */
+int pfx_ndplan_split_stage(pfx_ndplan plan, pfx_plan *out);
+
 /* ------------------------------------------- many uneven transforms */
 
 /*

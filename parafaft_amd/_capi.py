@@ -88,6 +88,9 @@ SIGNATURES = [
     ("pfx_ndplan_chunk", _i, [_vp, _ip]),
     ("pfx_ndplan_set_chunk", _i, [_vp, _i]),
     ("pfx_ndplan_chunk_stage", _i, [_vp, _i, _vpp]),
+    ("pfx_ndplan_split", _i, [_vp, _ip]),
+    ("pfx_ndplan_set_split", _i, [_vp, _i]),
+    ("pfx_ndplan_split_stage", _i, [_vp, _vpp]),
     ("pfx_batch_create", _i, [_vpp, _i, _i, _vp]),
     ("pfx_batch_tune", _i, [_vp, _i, _vp]),
     ("pfx_batch_execute", _i, [_vp, _i, _vp]),

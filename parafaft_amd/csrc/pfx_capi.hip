@@ -303,7 +303,19 @@ struct pfx_ndplan_t {
   int chunk = 0;
   int64_t plane_elems = 0;        // complex elements per axis-0 plane
   std::vector<pfx_plan> cstages;
+  // Split axis 0 (C2C, chunked, n0 = q G; pfx_planerow.hip): per chunk, the
+  // G-point DFTs along axis 0 fused with the last axis (into the scratch),
+  // then the middle axes on the scratch; then the q-point DFTs along axis 0
+  // from the scratch into the array.  0 = off.
+  int split = 0;
+  void *split_buf = nullptr;      // scratch: one whole array
+  void *split_tw = nullptr;       // W_n0^i, i < n0
+  void *split_ptw = nullptr;      // per-pass table of the fused kernel's last-axis radices
+  pfx_plan split_col = nullptr;   // the q-point pass
 };
+
+static int split_set(pfx_ndplan p, int g);
+static void split_free(pfx_ndplan p);
 
 extern "C" {
 
@@ -1579,9 +1591,29 @@ int pfx_ndplan_chunk_stage(pfx_ndplan p, int stage, pfx_plan *out) {
   return PFX_SUCCESS;
 }
 
+int pfx_ndplan_split(pfx_ndplan p, int *g) {
+  if (!p || !g) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  *g = p->split;
+  return PFX_SUCCESS;
+}
+
+int pfx_ndplan_set_split(pfx_ndplan p, int g) {
+  if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
+  if (g < 0) return fail(PFX_ERR_INVALID_ARG, "split factor must be >= 0");
+  return split_set(p, g);
+}
+
+int pfx_ndplan_split_stage(pfx_ndplan p, pfx_plan *out) {
+  if (!p || !out) return fail(PFX_ERR_INVALID_ARG, "null argument");
+  if (p->split == 0) return fail(PFX_ERR_INVALID_ARG, "axis 0 is not split");
+  *out = p->split_col;
+  return PFX_SUCCESS;
+}
+
 int pfx_ndplan_set_chunk(pfx_ndplan p, int planes) {
   if (!p) return fail(PFX_ERR_INVALID_ARG, "plan is null");
   if (planes == p->chunk) return PFX_SUCCESS;
+  split_free(p);  // the split schedule is made for one chunk size
   if (planes != 0 && (p->ndim < 3 || planes < 1 || planes >= p->shape[0] ||
                       p->shape[0] % planes != 0))
     return fail(PFX_ERR_INVALID_ARG, "chunk must divide n0 = %d and be below it (ndim >= 3)",
@@ -1593,6 +1625,110 @@ int pfx_ndplan_set_chunk(pfx_ndplan p, int planes) {
   std::vector<int64_t> cs(p->shape.begin(), p->shape.end());
   if (p->kind == PFX_R2C) cs[p->ndim - 1] = p->shape[p->ndim - 1] / 2 + 1;
   return make_chunk_plans(p, cs, planes);
+}
+
+// The split-axis-0 schedule (see pfx_ndplan_t::split).  Forward: per chunk of
+// P planes (P / G values of x1), the fused pass from the array into planes
+// [c P, (c + 1) P) of the scratch and the middle axes there; then the q-point
+// pass from the scratch into the array, with the scale.  Backward: the
+// q-point pass from the array into the scratch; per chunk the middle axes,
+// then the fused pass (twiddle before its DFT) into the array, with the scale.
+static int split_run(pfx_ndplan p, int direction, void *data, double scale, hipStream_t s) {
+  const int D = p->ndim;
+  const int bp = base_prec(p->precision);
+  const int64_t esz = bp ? 16 : 8;
+  const int G = p->split;
+  const int64_t q = p->shape[0] / G;
+  const int nchunks = p->shape[0] / p->chunk;
+  const size_t chunk_bytes = (size_t)(p->chunk * p->plane_elems * esz);
+  const bool fwd = direction == PFX_FORWARD;
+  PlaneRowArgs a;
+  a.src = fwd ? data : p->split_buf;
+  a.dst = fwd ? p->split_buf : data;
+  a.tw = p->split_tw;
+  a.ptw = p->split_ptw;
+  a.plane = p->plane_elems;
+  a.rows = p->plane_elems / p->shape[D - 1];
+  a.nx1 = p->chunk / G;
+  a.in_a = fwd ? 1 : G;
+  a.in_b = fwd ? q : 1;
+  a.out_a = fwd ? G : 1;
+  a.out_b = fwd ? 1 : q;
+  a.tw_pre = fwd ? 0 : 1;
+  a.backward = fwd ? 0 : 1;
+  a.use_scale = (!fwd && scale != 1.0) ? 1 : 0;
+  a.scale = fwd ? 1.0 : scale;
+  if (!fwd) {
+    int rc = launch_tiled(p->split_col, direction, p->split_buf, data, s);
+    if (rc != PFX_SUCCESS) return rc;
+  }
+  for (int c = 0; c < nchunks; ++c) {
+    char *base = static_cast<char *>(p->split_buf) + (size_t)c * chunk_bytes;
+    a.x1_begin = (int64_t)c * a.nx1;
+    if (fwd) PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
+    for (int k = 1; k < D - 1; ++k) {
+      const int st = fwd ? k : D - 1 - k;
+      int rc = pfx_execute_scaled(p->cstages[st], direction, base, 1.0, s);
+      if (rc != PFX_SUCCESS) return rc;
+    }
+    if (!fwd) PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
+  }
+  if (fwd) return launch_tiled(p->split_col, direction, data, p->split_buf, s, scale);
+  return PFX_SUCCESS;
+}
+
+static void split_free(pfx_ndplan p) {
+  if (p->split_col) pfx_plan_destroy(p->split_col);
+  if (p->split_buf) hipFree(p->split_buf);
+  if (p->split_tw) hipFree(p->split_tw);
+  if (p->split_ptw) hipFree(p->split_ptw);
+  p->split_col = nullptr;
+  p->split_buf = p->split_tw = p->split_ptw = nullptr;
+  p->split = 0;
+}
+
+// Turns the split-axis-0 schedule on with G planes per fused tile (the
+// scratch, tables and q-point plan are made here), or off (g = 0).
+static int split_set(pfx_ndplan p, int g) {
+  if (g == p->split) return PFX_SUCCESS;
+  split_free(p);
+  if (g == 0) return PFX_SUCCESS;
+  const int D = p->ndim;
+  const int bp = base_prec(p->precision);
+  Radices rad;
+  if (p->kind != PFX_C2C || (p->precision != PFX_F32 && p->precision != PFX_F64))
+    return fail(PFX_ERR_UNSUPPORTED, "split axis 0: C2C FP32/FP64 plans only");
+  if (D < 3 || p->chunk == 0)
+    return fail(PFX_ERR_UNSUPPORTED, "split axis 0 needs the chunked schedule (ndim >= 3)");
+  if (g < 2 || p->shape[0] % g != 0 || p->chunk % g != 0)
+    return fail(PFX_ERR_INVALID_ARG, "split factor %d must divide n0 = %d and the chunk (%d planes)",
+                g, p->shape[0], p->chunk);
+  if (!plane_row_radices(bp, p->shape[D - 1], g, &rad))
+    return fail(PFX_ERR_UNSUPPORTED, "no fused split-axis kernel for last-axis length %d, %d planes",
+                p->shape[D - 1], g);
+  const int64_t esz = bp ? 16 : 8;
+  const int64_t q = p->shape[0] / g;
+  int rc = pfx_plan_create_c2c_many(&p->split_col, p->precision, q, p->plane_elems,
+                                    (int64_t)g * p->plane_elems, 1, g, p->plane_elems);
+  if (rc != PFX_SUCCESS) return rc;
+  if (!p->split_col->kernel || p->split_col->lp) {
+    split_free(p);
+    return fail(PFX_ERR_UNSUPPORTED, "split axis 0: no tiled kernel for the %lld-point pass",
+                (long long)q);
+  }
+  const size_t bytes = (size_t)p->shape[0] * (size_t)p->plane_elems * (size_t)esz;
+  hipError_t e = hipMalloc(&p->split_buf, bytes);
+  if (e == hipSuccess) e = hipMalloc(&p->split_tw, (size_t)p->shape[0] * esz);
+  if (e == hipSuccess) e = hipMalloc(&p->split_ptw, (size_t)p->shape[D - 1] * esz);
+  if (e == hipSuccess) e = make_twiddles(bp, p->shape[0], p->split_tw, nullptr);
+  if (e == hipSuccess) e = make_pass_twiddles(bp, rad, p->split_ptw, nullptr);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  if (e != hipSuccess) {
+    split_free(p);
+    return hip_fail(e, "split axis 0 scratch");
+  }
+  p->split = g;
+  return PFX_SUCCESS;
 }
 
 // One N-D transform.  `real` (R2C plans only): the unpadded real array, read
@@ -1611,6 +1747,8 @@ static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_str
       return execute_impl(pl, direction, base, sc, stream, real_base, nlast);
     return pfx_execute_scaled(pl, direction, base, sc, stream);
   };
+  if (p->split > 0 && p->chunk > 0) return split_run(p, direction, data, scale,
+                                                     static_cast<hipStream_t>(stream));
   if (p->chunk > 0) {
     const int64_t esz = base_prec(p->precision) ? 16 : 8;
     const int nchunks = p->shape[0] / p->chunk;
@@ -1679,10 +1817,72 @@ int pfx_ndplan_execute_r2c_oop(pfx_ndplan p, int direction, void *real, void *pa
 
 int pfx_ndplan_destroy(pfx_ndplan p) {
   if (!p) return PFX_SUCCESS;
+  split_free(p);
   for (pfx_plan s : p->stages) pfx_plan_destroy(s);
   for (pfx_plan s : p->cstages) pfx_plan_destroy(s);
   delete p;
   return PFX_SUCCESS;
+}
+
+// Times the chunked forward against the split-axis-0 schedule for every
+// factor G with a fused kernel (each with its q-point pass tuned) and keeps
+// a split only if it is at least 2% faster (so timing noise does not trade
+// one schedule for the other); `force`: the best split in any case.
+// PFX_ND_SPLIT=G restricts the candidates to one factor.  A scratch that
+// cannot be allocated leaves the chunked schedule.
+static int split_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream, bool force) {
+  if (p->kind != PFX_C2C || p->chunk == 0 || p->ndim < 3 ||
+      (p->precision != PFX_F32 && p->precision != PFX_F64))
+    return PFX_SUCCESS;
+  const int D = p->ndim;
+  const int bp = base_prec(p->precision);
+  std::vector<int> gs;
+  const char *only = getenv("PFX_ND_SPLIT");
+  for (int g : {4, 8, 16}) {
+    if (only && *only && atoi(only) != g) continue;
+    if (p->shape[0] % g == 0 && p->chunk % g == 0 && plane_row_radices(bp, p->shape[D - 1], g, nullptr))
+      gs.push_back(g);
+  }
+  if (gs.empty()) return PFX_SUCCESS;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipEvent_t a, b;
+  PFX_HIP(hipEventCreate(&a), "hipEventCreate");
+  PFX_HIP(hipEventCreate(&b), "hipEventCreate");
+  const int reps = flag == PFX_PLAN_PATIENT ? 6 : 3;
+  auto time_fwd = [&](float *ms) -> int {
+    int rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);  // warm-up
+    hipEventRecord(a, s);
+    for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r)
+      rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);
+    hipEventRecord(b, s);
+    if (hipEventSynchronize(b) != hipSuccess) return PFX_ERR_HIP;
+    hipEventElapsedTime(ms, a, b);
+    return rc;
+  };
+  split_free(p);
+  float best = -1.f;
+  int best_g = 0, rc = PFX_SUCCESS;
+  if (!force) rc = time_fwd(&best);
+  for (int g : gs) {
+    if (rc != PFX_SUCCESS) break;
+    if (split_set(p, g) != PFX_SUCCESS) continue;  // no scratch: stay chunked
+    rc = pfx_plan_tune(p->split_col, flag, scratch, stream);
+    float ms = 0.f;
+    if (rc == PFX_SUCCESS) rc = time_fwd(&ms);
+    const float need = best_g == 0 && !force ? 0.98f * best : best;
+    if (rc == PFX_SUCCESS && (best < 0.f || ms < need)) {
+      best = ms;
+      best_g = g;
+    }
+    split_free(p);
+  }
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  if (rc != PFX_SUCCESS) return rc;
+  if (best_g == 0) return PFX_SUCCESS;
+  rc = split_set(p, best_g);
+  if (rc == PFX_SUCCESS) rc = pfx_plan_tune(p->split_col, flag, scratch, stream);
+  return rc;
 }
 
 int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
@@ -1765,11 +1965,14 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
     hipEventDestroy(a);
     hipEventDestroy(b);
   }
-  // PFX_ND_SCHEDULE=chunked|stagewise overrides the comparison below (for
-  // measurement, e.g. counter runs that serialise dispatches)
+  // PFX_ND_SCHEDULE=chunked|stagewise|split overrides the comparisons below
+  // (for measurement, e.g. counter runs that serialise dispatches): chunked
+  // keeps the chunks without a split axis 0, split keeps the chunks and the
+  // fastest split
   const char *forced = getenv("PFX_ND_SCHEDULE");
   if (forced && !strcmp(forced, "stagewise")) return pfx_ndplan_set_chunk(p, 0);
-  if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE && !(forced && !strcmp(forced, "chunked"))) {
+  const bool keep_chunks = forced && (!strcmp(forced, "chunked") || !strcmp(forced, "split"));
+  if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE && !keep_chunks) {
     // keep the chunked schedule only if it times faster than stage-by-stage
     hipStream_t s = static_cast<hipStream_t>(stream);
     hipEvent_t a, b;
@@ -1794,6 +1997,8 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
     if (rc != PFX_SUCCESS) return rc;
     if (ms[0] <= ms[1]) return pfx_ndplan_set_chunk(p, 0);
   }
+  if (flag != PFX_PLAN_ESTIMATE && !(forced && !strcmp(forced, "chunked")))
+    return split_tune(p, flag, scratch, stream, forced && !strcmp(forced, "split"));
   return PFX_SUCCESS;
 }
 

@@ -238,4 +238,28 @@ struct RealFullArgs {
 hipError_t launch_real_full(int prec, int which, void *rows, void *scr, const RealFullArgs &a,
                             hipStream_t s);
 
+// Split axis 0 of the one-GPU N-D plan (pfx_planerow.hip), n0 = q G: for
+// x1 in [x1_begin, x1_begin + nx1) and every row y of the planes, the G-point
+// DFTs along the planes x1 in_a + m in_b (m < G) of row y, the twiddle
+// W_n0^(x1 m) (tw_pre: on input m before the DFT, else on output m), and
+// the FFT of each row along the last axis, stored to planes x1 out_a +
+// m out_b.  A plane holds `plane` elements, `rows` rows of the last axis.
+struct PlaneRowArgs {
+  const void *src;
+  void *dst;
+  const void *tw;   // W_n0^i, i < n0
+  const void *ptw;  // per-pass table of the last axis' radices (plane_row_radices)
+  int64_t plane, rows;
+  int64_t x1_begin, nx1;
+  int64_t in_a, in_b, out_a, out_b;
+  int tw_pre;
+  int backward;
+  int use_scale;
+  double scale;
+};
+// The last-axis radices of the fused kernel for length n and G planes
+// (prec: 0 FP32, 1 FP64); false: none built.
+bool plane_row_radices(int prec, int64_t n, int g, Radices *rad);
+hipError_t launch_plane_row(int prec, int64_t n, int g, const PlaneRowArgs &a, hipStream_t s);
+
 }  // namespace pfx
