@@ -437,6 +437,9 @@ def main():
     ap.add_argument("--no-extra", action="store_true",
                     help="cube workload: skip the 512^3, pencil-share and uneven-batch cases "
                          "reported beside the headline")
+    ap.add_argument("--schedule", default="tuned", choices=["tuned", "chunked", "split"],
+                    help="cube workload: the N-D schedule; tuned = the planner's choice, "
+                         "chunked = no split axis 0, split = the fastest split axis 0")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a plain device copy of 4 GiB (the library's 2D-copy "
                          "kernel) five times, to scale rocprof FETCH/WRITE_SIZE counters")
@@ -498,24 +501,31 @@ def main():
     A.check(L.pfx_ndplan_create(C.byref(nd), prec, A.PFX_C2C, 3, shape), "pfx_ndplan_create")
     flag = {"estimate": A.PFX_PLAN_ESTIMATE, "measure": A.PFX_PLAN_MEASURE,
             "patient": A.PFX_PLAN_PATIENT}[args.plan]
+    if args.schedule != "tuned":
+        os.environ["PFX_ND_SCHEDULE"] = args.schedule
     if flag != A.PFX_PLAN_ESTIMATE:
         log(f"[bench] planning ({args.plan}) on the data buffer")
         A.check(L.pfx_ndplan_tune(nd.value, flag, x.data_ptr(), sp), "pfx_ndplan_tune")
+    if args.schedule != "tuned":
+        os.environ.pop("PFX_ND_SCHEDULE", None)  # the headline plan only
         torch.cuda.synchronize()
         g.manual_seed(20241203)   # planning destroyed the buffer: regenerate the input
         xr = torch.view_as_real(x.view(-1))
         xr.copy_(torch.rand(xr.shape, device=dev, generator=g, dtype=rdt) * 2 - 1)
         torch.cuda.synchronize()
-    # The timed steps below run the chunked schedule part by part from its
-    # stage plans; a split axis 0 (chosen by the planner only where it is
-    # faster, e.g. 2048^3) has no such parts, so it is turned off here and
-    # the line says so.
+    # A split axis 0 (where the planner finds it faster, e.g. 2048^3, or
+    # forced by --schedule split) runs as whole forwards; its q-point pass is
+    # timed alone for the roofline.  Otherwise the timed steps run the chunked
+    # (or stage-by-stage) schedule part by part from its stage plans.
     split_c = C.c_int()
     A.check(L.pfx_ndplan_split(nd.value, C.byref(split_c)), "pfx_ndplan_split")
-    split_tuned = split_c.value
-    if split_tuned:
-        log(f"[bench] planner chose a split axis 0 (G = {split_tuned}); timing the chunked parts")
-        A.check(L.pfx_ndplan_set_split(nd.value, 0), "pfx_ndplan_set_split")
+    split_g = split_c.value
+    split_h = None
+    if split_g:
+        h = C.c_void_p()
+        A.check(L.pfx_ndplan_split_stage(nd.value, C.byref(h)), "pfx_ndplan_split_stage")
+        split_h = h.value
+        log(f"[bench] split axis 0: G = {split_g}")
     stages = []
     for s in range(D):
         h = C.c_void_p()
@@ -538,7 +548,8 @@ def main():
         cstages.append(h.value)
     chunk_bytes = chunk * n * n * esz
     log(f"[bench] schedule: " + (f"axes 2+1 in chunks of {chunk} planes, then axis 0"
-                                 if chunk else "stage by stage"))
+                                 if chunk else "stage by stage")
+        + (f", axis 0 split ({split_g}-point DFTs fused into the row pass)" if split_g else ""))
 
     # ---- accuracy (outside the timed region): FP32 vs FP64 on the same input
     def accuracy(ndh):
@@ -573,6 +584,8 @@ def main():
 
     # timed parts: the D stages, or (chunked) axes 2+1 together and axis 0
     parts = [[h] for h in stages] if not chunk else [None, [stages[D - 1]]]
+    if split_g:
+        parts = ["nd"]
 
     def step(events=None, st=None):
         st = st or sp
@@ -580,7 +593,9 @@ def main():
         for s, hs in enumerate(parts):
             if events is not None:
                 events[s][0].record(stream)
-            if hs is None:
+            if hs == "nd":
+                A.check(L.pfx_ndplan_execute(nd.value, A.PFX_FORWARD, base, 1.0, st), "fwd")
+            elif hs is None:
                 for c in range(n // chunk):
                     for h in cstages:
                         rc = L.pfx_execute(h, A.PFX_FORWARD, base + c * chunk_bytes, st)
@@ -675,7 +690,21 @@ def main():
     tbs = alg_bytes / (ms_per_step * 1e-3) / 1e12
     # dominant kernel: the single launch with the most time (axis 0 when the
     # other axes run in chunks)
-    if chunk:
+    dom_name = None
+    if split_g:
+        # the q-point pass alone, in place on the array (same kernel and
+        # geometry as from the scratch), K launches between two events
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        A.check(L.pfx_execute(split_h, A.PFX_FORWARD, x.data_ptr(), sp), "q-point pass")
+        e0.record(stream)
+        for _ in range(args.steps):
+            A.check(L.pfx_execute(split_h, A.PFX_FORWARD, x.data_ptr(), sp), "q-point pass")
+        e1.record(stream)
+        torch.cuda.synchronize()
+        dom, dom_ms = D - 1, e0.elapsed_time(e1) / args.steps
+        dom_name = describe(split_h)
+        part_ms = [part_ms[0], dom_ms]
+    elif chunk:
         dom, dom_ms = D - 1, part_ms[1]
     else:
         dom = max(range(D), key=lambda s: part_ms[s])
@@ -692,7 +721,7 @@ def main():
                 t = json.load(f)
             st = t.get("stages", {}).get(str(dom), {})
             # counters of exactly the kernel this run times; otherwise none
-            if st.get("plan_kernel") == names[dom]:
+            if st.get("plan_kernel") == (dom_name or names[dom]):
                 traffic = st.get("hbm_bytes_per_launch")
         except Exception:
             traffic = None
@@ -723,11 +752,16 @@ def main():
                                f"complex<{'double' if prec == A.PFX_F64 else 'float'}>, one GPU "
                                "(one-rank fast path: "
                                + (f"axes 2 and 1 in chunks of {chunk} planes through the "
-                                  "Infinity Cache, then axis 0)" if chunk else
+                                  "Infinity Cache, then axis 0)" if chunk and not split_g else
+                                  f"axis 0 split as {split_g} x {n // max(split_g, 1)}: "
+                                  f"{split_g}-point DFTs fused into the row pass of each chunk "
+                                  "of planes, middle axis on the chunk, then the "
+                                  f"{n // max(split_g, 1)}-point pass)" if split_g else
                                   "one launch per axis)"),
                    "grid": [n, n, n], "precision": args.precision, "planning": args.plan,
                    "chunk_planes": chunk,
-                   "split_axis0_chosen_by_planner": split_tuned,
+                   "schedule": args.schedule,
+                   "split_axis0": split_g,
                    "stage_kernels": names,
                    "chunk_kernels": [describe(h) for h in cstages]},
         "hbm_tbs_algorithmic": round(tbs, 3),
@@ -736,12 +770,14 @@ def main():
         "event_median_ms_per_step": round(median_ms, 4),
         "graph_replay_ms_per_step": round(graph_ms, 4) if graph_ms else None,
         "forward_backward_ms": fb_ms,
-        "part_ms": ({"axes 2+1 (chunked)": round(part_ms[0], 4), "axis 0": round(part_ms[1], 4)}
+        "part_ms": ({"forward (split axis 0)": round(part_ms[0], 4),
+                     "q-point pass (alone)": round(part_ms[1], 4)} if split_g else
+                    {"axes 2+1 (chunked)": round(part_ms[0], 4), "axis 0": round(part_ms[1], 4)}
                     if chunk else {f"axis {D - 1 - s}": round(t, 4) for s, t in enumerate(part_ms)}),
         "accuracy": acc,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "kernel": names[dom],
+                     "traffic": traffic, "kernel": dom_name or names[dom],
                      "algorithmic_bytes_per_launch": per_launch},
         "cpu_baseline": cpu,
     }
@@ -781,7 +817,7 @@ def main():
     # ---- split-precision MFMA forms on the same grid and input (the metric's
     # "split vs plain"): planned the same way, checked against the same FP64
     # path, timed over the same K forwards
-    if prec == A.PFX_F32 and not args.no_split:
+    if prec == A.PFX_F32 and not args.no_split and not split_g:
         split = {}
         for mode in ("f32_split_f16", "f32_split_bf16"):
             nds = C.c_void_p()

@@ -357,7 +357,9 @@ int pfx_ndplan_stage(pfx_ndplan plan, int stage, pfx_plan *out);
  * into the array.  Axis 0's column runs get q / n0 of the rows to cross
  * instead of n0, at the cost of one array-sized scratch the plan holds.
  * g = 0 turns it off; pfx_ndplan_tune (Measure/Patient) keeps it only if it
- * times faster than the chunked schedule.  Changing the chunk turns it off.
+ * times at least 2% faster than the chunked schedule.  Changing the chunk
+ * turns it off.  The scratch belongs to the plan: executions of one split
+ * plan must not overlap (on different streams).
  * No counterpart in the reference (its stage loop is
  * parafaft_c2c.hpp:539-602).
  */

@@ -1725,6 +1725,9 @@ static int split_set(pfx_ndplan p, int g) {
   if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
   if (e != hipSuccess) {
     split_free(p);
+    // an allocation that did not fit stays the runtime's last error; clear it
+    // so that the next launch check does not report it as its own
+    (void)hipGetLastError();
     return hip_fail(e, "split axis 0 scratch");
   }
   p->split = g;
