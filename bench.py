@@ -506,13 +506,13 @@ def main():
     if flag != A.PFX_PLAN_ESTIMATE:
         log(f"[bench] planning ({args.plan}) on the data buffer")
         A.check(L.pfx_ndplan_tune(nd.value, flag, x.data_ptr(), sp), "pfx_ndplan_tune")
-    if args.schedule != "tuned":
-        os.environ.pop("PFX_ND_SCHEDULE", None)  # the headline plan only
         torch.cuda.synchronize()
         g.manual_seed(20241203)   # planning destroyed the buffer: regenerate the input
         xr = torch.view_as_real(x.view(-1))
         xr.copy_(torch.rand(xr.shape, device=dev, generator=g, dtype=rdt) * 2 - 1)
         torch.cuda.synchronize()
+    if args.schedule != "tuned":
+        os.environ.pop("PFX_ND_SCHEDULE", None)  # the headline plan only
     # A split axis 0 (where the planner finds it faster, e.g. 2048^3, or
     # forced by --schedule split) runs as whole forwards; its q-point pass is
     # timed alone for the roofline.  Otherwise the timed steps run the chunked
