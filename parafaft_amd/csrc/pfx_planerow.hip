@@ -52,13 +52,19 @@ __global__ void __launch_bounds__(TPT * G) plane_row_kernel(PlaneRowArgs a) {
   const V2<T>* src = static_cast<const V2<T>*>(a.src) + (x1 * a.in_a * a.plane + y * N);
   const int64_t ls = a.in_b * a.plane;  // between the tile's G rows
   const V2<T>* w = static_cast<const V2<T>*>(a.tw);
+  // Streaming loads: the source is not read again before the last pass
+  // rewrites it, so its lines need not take Infinity Cache space from the
+  // scratch chunk this kernel writes and the middle axes read next.
+  typedef T E2 __attribute__((ext_vector_type(2)));
+  using GE2 = __attribute__((address_space(1))) E2;
   V2<T> v[E][G];
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int z = tid + e * NT;
 #pragma unroll
     for (int m = 0; m < G; ++m) {
-      const V2<T> x = src[m * ls + z];
+      const E2 x2 = __builtin_nontemporal_load((const GE2*)(src + (m * ls + z)));
+      const V2<T> x = mk<T>(x2.x, x2.y);
       v[e][m] = bw ? swp<T>(x) : x;
     }
   }
