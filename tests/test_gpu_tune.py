@@ -60,6 +60,10 @@ GEOMS = [  # (prec, n, batch, stride, dist, outer, outer_dist)
     ("f32", 1024, 40, 40, 1, 3, 1024 * 40), ("f32", 2048, 50, 1, 2048, 1, 0),
     ("f32", 2048, 33, 33, 1, 1, 0), ("f32", 4096, 20, 20, 1, 1, 0), ("f32", 512, 70, 70, 1, 1, 0),
     ("f32", 256, 129, 129, 1, 1, 0), ("f64", 1024, 40, 40, 1, 1, 0), ("f64", 1024, 33, 1, 1024, 1, 0),
+    # short batch-contiguous lengths (the wide 128- and 256-column tiles of
+    # 64 and 128 points; partial last tiles; a two-level batch)
+    ("f32", 128, 300, 300, 1, 1, 0), ("f32", 128, 130, 130, 1, 2, 128 * 130),
+    ("f32", 64, 520, 520, 1, 1, 0),
     ("f32_split_f16", 1024, 50, 50, 1, 1, 0), ("f32_split_bf16", 1024, 50, 1, 1024, 1, 0),
 ]
 
