@@ -285,8 +285,14 @@ bool is_plan_key(const char *s) {
   int fields = sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld%c", &pr, &kd, &n, &i, &o,
                       &st, &d, &dout, &tail);
   if (fields == 8) return true;
-  return sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld chunk%d of%d%c", &pr, &kd, &n,
-                &i, &o, &st, &d, &dout, &P, &n0, &tail) == 10;
+  if (sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld chunk%d of%d%c", &pr, &kd, &n, &i,
+             &o, &st, &d, &dout, &P, &n0, &tail) == 10)
+    return true;
+  // a backward chunk plan's in-context choice
+  char bwd[4];
+  return sscanf(s, "p%d k%d n%lld i%lld o%lld s%lld d%lld do%lld chunk%d of%d %3s%c", &pr, &kd, &n,
+                &i, &o, &st, &d, &dout, &P, &n0, bwd, &tail) == 11 &&
+         !strcmp(bwd, "bwd");
 }
 
 }  // namespace
@@ -303,6 +309,10 @@ struct pfx_ndplan_t {
   int chunk = 0;
   int64_t plane_elems = 0;        // complex elements per axis-0 plane
   std::vector<pfx_plan> cstages;
+  // The same chunk plans for backward transforms, whose chunks run the middle
+  // axes before the last (reading HBM first, the cache second): tuned in the
+  // backward schedule, so their kernels may differ from the forward ones.
+  std::vector<pfx_plan> bstages;
   // Split axis 0 (C2C, chunked, n0 = q G; pfx_planerow.hip): per chunk, the
   // G-point DFTs along axis 0 fused with the last axis (into the scratch),
   // then the middle axes on the scratch; then the q-point DFTs along axis 0
@@ -1461,8 +1471,10 @@ int64_t env_mb_nd(const char *name, int64_t dflt) {
   return v && *v ? (int64_t)atoll(v) : dflt;
 }
 
-// Plans of stages 0..D-2 restricted to P axis-0 planes (+ the C2R on P planes).
-int make_chunk_plans(pfx_ndplan p, const std::vector<int64_t> &cs, int P) {
+// Plans of stages 0..D-2 restricted to P axis-0 planes (+ the C2R on P
+// planes), into `out`.
+int make_chunk_list(pfx_ndplan p, const std::vector<int64_t> &cs, int P,
+                    std::vector<pfx_plan> &out) {
   const int ndim = p->ndim;
   const int prec = p->precision;
   for (int s = 0; s < ndim - 1; ++s) {
@@ -1479,7 +1491,7 @@ int make_chunk_plans(pfx_ndplan p, const std::vector<int64_t> &cs, int P) {
     else
       rc = pfx_plan_create_c2c_many(&pl, prec, cs[axis], trail, trail, 1, lead, cs[axis] * trail);
     if (rc != PFX_SUCCESS) return rc;
-    p->cstages.push_back(pl);
+    out.push_back(pl);
   }
   if (p->kind == PFX_R2C) {
     int64_t lead = P;
@@ -1487,8 +1499,16 @@ int make_chunk_plans(pfx_ndplan p, const std::vector<int64_t> &cs, int P) {
     pfx_plan pl = nullptr;
     int rc = pfx_plan_create_c2r(&pl, prec, p->shape[ndim - 1], lead, 1, 2 * cs[ndim - 1]);
     if (rc != PFX_SUCCESS) return rc;
-    p->cstages.push_back(pl);
+    out.push_back(pl);
   }
+  return PFX_SUCCESS;
+}
+
+// The forward chunk plans and their backward twins.
+int make_chunk_plans(pfx_ndplan p, const std::vector<int64_t> &cs, int P) {
+  int rc = make_chunk_list(p, cs, P, p->cstages);
+  if (rc == PFX_SUCCESS) rc = make_chunk_list(p, cs, P, p->bstages);
+  if (rc != PFX_SUCCESS) return rc;
   p->chunk = P;
   return PFX_SUCCESS;
 }
@@ -1619,7 +1639,9 @@ int pfx_ndplan_set_chunk(pfx_ndplan p, int planes) {
     return fail(PFX_ERR_INVALID_ARG, "chunk must divide n0 = %d and be below it (ndim >= 3)",
                 p->ndim ? p->shape[0] : 0);
   for (pfx_plan c : p->cstages) pfx_plan_destroy(c);
+  for (pfx_plan c : p->bstages) pfx_plan_destroy(c);
   p->cstages.clear();
+  p->bstages.clear();
   p->chunk = 0;
   if (planes == 0) return PFX_SUCCESS;
   std::vector<int64_t> cs(p->shape.begin(), p->shape.end());
@@ -1668,7 +1690,7 @@ static int split_run(pfx_ndplan p, int direction, void *data, double scale, hipS
     if (fwd) PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
     for (int k = 1; k < D - 1; ++k) {
       const int st = fwd ? k : D - 1 - k;
-      int rc = pfx_execute_scaled(p->cstages[st], direction, base, 1.0, s);
+      int rc = pfx_execute_scaled(fwd ? p->cstages[st] : p->bstages[st], direction, base, 1.0, s);
       if (rc != PFX_SUCCESS) return rc;
     }
     if (!fwd) PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
@@ -1766,8 +1788,9 @@ static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_str
         for (int k = 0; k < D - 1; ++k) {
           // forward: stages 0..D-2; backward: D-2..0 (C2R last for R2C)
           const int st = direction == PFX_FORWARD ? k : D - 2 - k;
-          pfx_plan pl = p->cstages[st];
-          if (p->kind == PFX_R2C && st == 0 && direction == PFX_BACKWARD) pl = p->cstages[D - 1];
+          const std::vector<pfx_plan> &cl = direction == PFX_FORWARD ? p->cstages : p->bstages;
+          pfx_plan pl = cl[st];
+          if (p->kind == PFX_R2C && st == 0 && direction == PFX_BACKWARD) pl = cl[D - 1];
           const double sc = (k == D - 2) ? last_scale : 1.0;
           int rc = exec(pl, base, sc, rbase);
           if (rc != PFX_SUCCESS) return rc;
@@ -1811,7 +1834,7 @@ int pfx_ndplan_execute_r2c_oop(pfx_ndplan p, int direction, void *real, void *pa
   const int D = p->ndim;
   const pfx_plan real_stage = direction == PFX_FORWARD
                                   ? (p->chunk > 0 ? p->cstages[0] : p->stages[0])
-                                  : (p->chunk > 0 ? p->cstages[D - 1] : p->stages[D]);
+                                  : (p->chunk > 0 ? p->bstages[D - 1] : p->stages[D]);
   if (!real_stage->kernel)
     return fail(PFX_ERR_UNSUPPORTED,
                 "out-of-place R2C/C2R needs a tiled kernel (even length with a radix plan)");
@@ -1823,6 +1846,7 @@ int pfx_ndplan_destroy(pfx_ndplan p) {
   split_free(p);
   for (pfx_plan s : p->stages) pfx_plan_destroy(s);
   for (pfx_plan s : p->cstages) pfx_plan_destroy(s);
+  for (pfx_plan s : p->bstages) pfx_plan_destroy(s);
   delete p;
   return PFX_SUCCESS;
 }
@@ -1899,12 +1923,14 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
   // the whole chunked schedule per candidate (a chunk's short launches alone
   // are too noisy, and the best kernel for a pass that reads from the
   // Infinity Cache differs from the best for one that reads HBM).
-  for (size_t c = 0; c < p->cstages.size(); ++c) {
-    const size_t src = (p->kind == PFX_R2C && c + 1 == p->cstages.size()) ? p->ndim : c;
-    pfx_plan full = p->stages[src], part = p->cstages[c];
-    if (flag != PFX_PLAN_ESTIMATE && full->kernel && part->kernel) {
-      PFX_HIP(set_kernel(part, full->kernel), "chunk kernel");
-      part->xcd_remap = full->xcd_remap;
+  for (std::vector<pfx_plan> *list : {&p->cstages, &p->bstages}) {
+    for (size_t c = 0; c < list->size(); ++c) {
+      const size_t src = (p->kind == PFX_R2C && c + 1 == list->size()) ? p->ndim : c;
+      pfx_plan full = p->stages[src], part = (*list)[c];
+      if (flag != PFX_PLAN_ESTIMATE && full->kernel && part->kernel) {
+        PFX_HIP(set_kernel(part, full->kernel), "chunk kernel");
+        part->xcd_remap = full->xcd_remap;
+      }
     }
   }
   if (p->chunk > 0 && flag != PFX_PLAN_ESTIMATE) {
@@ -1913,23 +1939,24 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
     PFX_HIP(hipEventCreate(&a), "hipEventCreate");
     PFX_HIP(hipEventCreate(&b), "hipEventCreate");
     const int reps = flag == PFX_PLAN_PATIENT ? 4 : 2;
-    auto time_schedule = [&](float *ms) -> int {
-      int rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);  // warm-up
+    auto time_schedule = [&](int direction, float *ms) -> int {
+      int rc = pfx_ndplan_execute(p, direction, scratch, 1.0, stream);  // warm-up
       hipEventRecord(a, s);
       for (int r = 0; r < reps && rc == PFX_SUCCESS; ++r)
-        rc = pfx_ndplan_execute(p, PFX_FORWARD, scratch, 1.0, stream);
+        rc = pfx_ndplan_execute(p, direction, scratch, 1.0, stream);
       hipEventRecord(b, s);
       if (hipEventSynchronize(b) != hipSuccess) return PFX_ERR_HIP;
       hipEventElapsedTime(ms, a, b);
       return rc;
     };
-    for (int st = 0; st + 1 < p->ndim; ++st) {
-      pfx_plan part = p->cstages[st];
-      if (!part->kernel) continue;
-      // the in-context choice is cached under the chunk plan's key with a
-      // suffix (a cache file replays it, as for single plans)
-      const std::string ctx_key = plan_key(part) + " chunk" + std::to_string(p->chunk) +
-                                  " of" + std::to_string(p->shape[0]);
+    // Each chunk stage a direction runs, re-tuned in that direction's whole
+    // chunked schedule; the in-context choice is cached under the chunk
+    // plan's key with a suffix (a cache file replays it, as for single plans).
+    auto tune_in_context = [&](pfx_plan part, int direction) -> int {
+      if (!part->kernel) return PFX_SUCCESS;
+      const std::string ctx_key = plan_key(part) + " chunk" + std::to_string(p->chunk) + " of" +
+                                  std::to_string(p->shape[0]) +
+                                  (direction == PFX_BACKWARD ? " bwd" : "");
       {
         std::lock_guard<std::mutex> lock(g_tune_mu);
         auto it = g_tune.find(ctx_key);
@@ -1939,7 +1966,7 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
             kernel_fits_plan(part, part->kernel, e, order)) {
           PFX_HIP(set_kernel(part, e), "chunk kernel");
           part->xcd_remap = order;
-          continue;
+          return PFX_SUCCESS;
         }
       }
       const KernelEntry *best = part->kernel;
@@ -1952,7 +1979,7 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
         for (int order : orders) {
           part->xcd_remap = order;
           float ms = 0.f;
-          if (time_schedule(&ms) != PFX_SUCCESS) continue;
+          if (time_schedule(direction, &ms) != PFX_SUCCESS) continue;
           if (best_ms < 0.f || ms < best_ms) {
             best_ms = ms;
             best = k;
@@ -1964,6 +1991,20 @@ int pfx_ndplan_tune(pfx_ndplan p, int flag, void *scratch, pfx_stream stream) {
       part->xcd_remap = best_order;
       std::lock_guard<std::mutex> lock(g_tune_mu);
       g_tune[ctx_key] = entry_id(best) + "|order=" + std::to_string(best_order);
+      return PFX_SUCCESS;
+    };
+    int rc = PFX_SUCCESS;
+    for (int st = 0; st + 1 < p->ndim && rc == PFX_SUCCESS; ++st)
+      rc = tune_in_context(p->cstages[st], PFX_FORWARD);
+    // backward: the same stages, except that an R2C plan's last axis runs
+    // its C2R plan
+    for (int st = 0; st + 1 < p->ndim && rc == PFX_SUCCESS; ++st)
+      rc = tune_in_context(p->bstages[p->kind == PFX_R2C && st == 0 ? p->ndim - 1 : st],
+                           PFX_BACKWARD);
+    if (rc != PFX_SUCCESS) {
+      hipEventDestroy(a);
+      hipEventDestroy(b);
+      return rc;
     }
     hipEventDestroy(a);
     hipEventDestroy(b);

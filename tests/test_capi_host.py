@@ -181,14 +181,19 @@ def test_tune_cache_load_counts_dropped_lines(tmp_path):
 
 def test_tune_cache_keeps_in_context_chunk_choices(tmp_path):
     """An N-D plan's in-context chunk choice is cached under the chunk
-    plan's key plus " chunkP ofN": such a line loads (host-only), a
-    truncated suffix is dropped."""
+    plan's key plus " chunkP ofN" (forward) or " chunkP ofN bwd" (backward
+    chunk plans): such lines load (host-only); a truncated or unknown suffix
+    is dropped."""
     from parafaft_amd import backend as B
     path = tmp_path / "cache.txt"
     val = "0|0|1024|1|32|wide-nt:32, 32|order=32"
-    path.write_text(f"p0 k0 n1024 i1024 o32 s1024 d1 do1048576 chunk32 of1024\t{val}\n"
-                    f"p0 k0 n1024 i1024 o32 s1024 d1 do1048576 chunk32\t{val}\n")
+    key = "p0 k0 n1024 i1024 o32 s1024 d1 do1048576"
+    path.write_text(f"{key} chunk32 of1024\t{val}\n"
+                    f"{key} chunk32\t{val}\n"
+                    f"{key} chunk32 of1024 bwd\t{val}\n"
+                    f"{key} chunk32 of1024 fwd\t{val}\n"
+                    f"{key} chunk32 of1024 bwdx\t{val}\n")
     B.tune_cache_clear()
-    assert B.tune_cache_load(str(path)) == 1
-    assert B.tune_cache_size() == 1
+    assert B.tune_cache_load(str(path)) == 3
+    assert B.tune_cache_size() == 2
     B.tune_cache_clear()
