@@ -1649,12 +1649,12 @@ int pfx_ndplan_set_chunk(pfx_ndplan p, int planes) {
   return make_chunk_plans(p, cs, planes);
 }
 
-// The split-axis-0 schedule (see pfx_ndplan_t::split).  Forward: per chunk of
-// P planes (P / G values of x1), the fused pass from the array into planes
+// The split-axis-0 schedule (see pfx_ndplan_t::split): per chunk of P planes
+// (P / G values of x1), the fused pass from the array into planes
 // [c P, (c + 1) P) of the scratch and the middle axes there; then the q-point
-// pass from the scratch into the array, with the scale.  Backward: the
-// q-point pass from the array into the scratch; per chunk the middle axes,
-// then the fused pass (twiddle before its DFT) into the array, with the scale.
+// pass from the scratch into the array, with the scale.  A backward runs the
+// same steps conjugated (each kernel's re/im swap), which inverts the whole
+// product in the same order.
 static int split_run(pfx_ndplan p, int direction, void *data, double scale, hipStream_t s) {
   const int D = p->ndim;
   const int bp = base_prec(p->precision);
@@ -1665,38 +1665,28 @@ static int split_run(pfx_ndplan p, int direction, void *data, double scale, hipS
   const size_t chunk_bytes = (size_t)(p->chunk * p->plane_elems * esz);
   const bool fwd = direction == PFX_FORWARD;
   PlaneRowArgs a;
-  a.src = fwd ? data : p->split_buf;
-  a.dst = fwd ? p->split_buf : data;
+  a.src = data;
+  a.dst = p->split_buf;
   a.tw = p->split_tw;
   a.ptw = p->split_ptw;
   a.plane = p->plane_elems;
   a.rows = p->plane_elems / p->shape[D - 1];
   a.nx1 = p->chunk / G;
-  a.in_a = fwd ? 1 : G;
-  a.in_b = fwd ? q : 1;
-  a.out_a = fwd ? G : 1;
-  a.out_b = fwd ? 1 : q;
-  a.tw_pre = fwd ? 0 : 1;
+  a.in_a = 1;
+  a.in_b = q;
+  a.out_a = G;
+  a.out_b = 1;
   a.backward = fwd ? 0 : 1;
-  a.use_scale = (!fwd && scale != 1.0) ? 1 : 0;
-  a.scale = fwd ? 1.0 : scale;
-  if (!fwd) {
-    int rc = launch_tiled(p->split_col, direction, p->split_buf, data, s);
-    if (rc != PFX_SUCCESS) return rc;
-  }
   for (int c = 0; c < nchunks; ++c) {
     char *base = static_cast<char *>(p->split_buf) + (size_t)c * chunk_bytes;
     a.x1_begin = (int64_t)c * a.nx1;
-    if (fwd) PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
+    PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
     for (int k = 1; k < D - 1; ++k) {
-      const int st = fwd ? k : D - 1 - k;
-      int rc = pfx_execute_scaled(fwd ? p->cstages[st] : p->bstages[st], direction, base, 1.0, s);
+      int rc = pfx_execute_scaled(fwd ? p->cstages[k] : p->bstages[k], direction, base, 1.0, s);
       if (rc != PFX_SUCCESS) return rc;
     }
-    if (!fwd) PFX_HIP(launch_plane_row(bp, p->shape[D - 1], G, a, s), "split-axis pass");
   }
-  if (fwd) return launch_tiled(p->split_col, direction, data, p->split_buf, s, scale);
-  return PFX_SUCCESS;
+  return launch_tiled(p->split_col, direction, data, p->split_buf, s, scale);
 }
 
 static void split_free(pfx_ndplan p) {
@@ -1774,6 +1764,13 @@ static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_str
   };
   if (p->split > 0 && p->chunk > 0) return split_run(p, direction, data, scale,
                                                      static_cast<hipStream_t>(stream));
+  // Axis order.  The transforms along different axes commute, and a C2C
+  // backward is every forward pass conjugated (each kernel's re/im swap), so
+  // a C2C backward runs the forward's order: last axis first, which lets a
+  // chunk's middle axes read what its row pass just wrote from the Infinity
+  // Cache.  R2C plans run backward in reverse (the C2R pass takes the
+  // complex half-spectrum and must come last).
+  const bool fwd_order = direction == PFX_FORWARD || p->kind == PFX_C2C;
   if (p->chunk > 0) {
     const int64_t esz = base_prec(p->precision) ? 16 : 8;
     const int nchunks = p->shape[0] / p->chunk;
@@ -1786,8 +1783,8 @@ static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_str
         char *base = static_cast<char *>(data) + (size_t)c * chunk_bytes;
         char *rbase = real ? static_cast<char *>(real) + (size_t)c * chunk_real_bytes : nullptr;
         for (int k = 0; k < D - 1; ++k) {
-          // forward: stages 0..D-2; backward: D-2..0 (C2R last for R2C)
-          const int st = direction == PFX_FORWARD ? k : D - 2 - k;
+          // forward order: stages 0..D-2; reverse: D-2..0 (C2R last for R2C)
+          const int st = fwd_order ? k : D - 2 - k;
           const std::vector<pfx_plan> &cl = direction == PFX_FORWARD ? p->cstages : p->bstages;
           pfx_plan pl = cl[st];
           if (p->kind == PFX_R2C && st == 0 && direction == PFX_BACKWARD) pl = cl[D - 1];
@@ -1798,7 +1795,7 @@ static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_str
       }
       return PFX_SUCCESS;
     };
-    if (direction == PFX_FORWARD) {
+    if (fwd_order) {
       int rc = run_chunks(1.0);
       if (rc != PFX_SUCCESS) return rc;
       return pfx_execute_scaled(p->stages[D - 1], direction, data, scale, stream);
@@ -1808,7 +1805,7 @@ static int nd_run(pfx_ndplan p, int direction, void *data, double scale, pfx_str
     return run_chunks(scale);
   }
   for (int k = 0; k < D; ++k) {
-    const int s = direction == PFX_FORWARD ? k : D - 1 - k;
+    const int s = fwd_order ? k : D - 1 - k;
     pfx_plan pl = p->stages[s];
     if (p->kind == PFX_R2C && s == 0 && direction == PFX_BACKWARD) pl = p->stages[D];
     const double sc = (k == D - 1) ? scale : 1.0;

@@ -241,9 +241,10 @@ hipError_t launch_real_full(int prec, int which, void *rows, void *scr, const Re
 // Split axis 0 of the one-GPU N-D plan (pfx_planerow.hip), n0 = q G: for
 // x1 in [x1_begin, x1_begin + nx1) and every row y of the planes, the G-point
 // DFTs along the planes x1 in_a + m in_b (m < G) of row y, the twiddle
-// W_n0^(x1 m) (tw_pre: on input m before the DFT, else on output m), and
-// the FFT of each row along the last axis, stored to planes x1 out_a +
-// m out_b.  A plane holds `plane` elements, `rows` rows of the last axis.
+// W_n0^(x1 m) on output m, and the FFT of each row along the last axis,
+// stored to planes x1 out_a + m out_b (backward: the same, conjugated through
+// the re/im swap).  A plane holds `plane` elements, `rows` rows of the last
+// axis.
 struct PlaneRowArgs {
   const void *src;
   void *dst;
@@ -252,10 +253,7 @@ struct PlaneRowArgs {
   int64_t plane, rows;
   int64_t x1_begin, nx1;
   int64_t in_a, in_b, out_a, out_b;
-  int tw_pre;
   int backward;
-  int use_scale;
-  double scale;
 };
 // The last-axis radices of the fused kernel for length n and G planes
 // (prec: 0 FP32, 1 FP64); false: none built.

@@ -17,10 +17,9 @@
 // one stride: an ordinary batch-contiguous plan from the scratch into the
 // caller's array.
 //
-// Backward (the inverse of the above, through the re/im swap that turns every
-// forward operation into its conjugate): the tile reads planes G x1 + k2,
-// multiplies input k2 by W_n0^(x1 k2) before the DFT, and stores output x2 to
-// plane x1 + q x2.  The plane maps are arguments, so one kernel serves both.
+// Backward: the same kernel with the re/im swap on its loads and stores, which
+// turns each of its steps (and the whole split schedule, run in the same
+// order) into its conjugate, the inverse DFT.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 #include "pfx_internal.hpp"
@@ -72,19 +71,12 @@ __global__ void __launch_bounds__(TPT * G) plane_row_kernel(PlaneRowArgs a) {
   V2<T> t[G];
 #pragma unroll
   for (int m = 1; m < G; ++m) t[m] = w[x1 * m];
-  const bool pre = a.tw_pre != 0;
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     const int z = tid + e * NT;
-    if (pre) {
-#pragma unroll
-      for (int m = 1; m < G; ++m) v[e][m] = cmul<T>(v[e][m], t[m]);
-    }
     bfly<T, G>(v[e]);
-    if (!pre) {
 #pragma unroll
-      for (int m = 1; m < G; ++m) v[e][m] = cmul<T>(v[e][m], t[m]);
-    }
+    for (int m = 1; m < G; ++m) v[e][m] = cmul<T>(v[e][m], t[m]);
 #pragma unroll
     for (int m = 0; m < G; ++m) lds[L::at(z, m)] = v[e][m];
   }
@@ -95,19 +87,13 @@ __global__ void __launch_bounds__(TPT * G) plane_row_kernel(PlaneRowArgs a) {
   auto ld = [&](int e) -> V2<T> { return lds[L::at(e, c)]; };
   struct Store {
     V2<T>* dst;
-    T scale;
-    bool use_scale, bw;
+    bool bw;
     __device__ void before() const {}
     __device__ void after() const {}
-    __device__ void operator()(int e, V2<T> v) const {
-      if (use_scale) v = scl<T>(v, scale);
-      if (bw) v = swp<T>(v);
-      dst[e] = v;
-    }
+    __device__ void operator()(int e, V2<T> v) const { dst[e] = bw ? swp<T>(v) : v; }
   };
   V2<T>* dst = static_cast<V2<T>*>(a.dst) + ((x1 * a.out_a + c * a.out_b) * a.plane + y * N);
-  FsRun<T, N, TPT, G, false, 1, Rs...>::run(
-      j, c, lds, stw, ld, Store{dst, T(a.scale), a.use_scale != 0, bw});
+  FsRun<T, N, TPT, G, false, 1, Rs...>::run(j, c, lds, stw, ld, Store{dst, bw});
 }
 
 template <typename T, int G, int TPT, int... Rs>
